@@ -1,0 +1,153 @@
+"""Benchmark of the Fermat squaring engine on one MI355X.
+
+Workload (the batched configuration): 256 candidates N = b^n - b^(n/2) + 1,
+n = 2^14 digits, b uniform in [2^18, 2^19) from a fixed seed, run through the
+base-2 PRP loop (x <- x^2 * 2^bit) in one persistent resident launch; a step is
+one squaring of all 256 candidates.  Beside it: the plain one-candidate engine
+at n = 2^14 (batched / plain ratio) and the single large candidate at n = 2^21
+(b = 65536 + 2u), with its HBM roofline, and the oracle timed on the host CPU.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import random
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_T = 39.3          # 256 CU x 64 lanes x 2.4 GHz, 32-bit ops (mul_hi/mul_lo full rate on gfx950)
+SEED = 20261203
+
+
+def mul32_alg(n):
+    """Algorithmic 32-bit multiplies per squaring: 9 n (log2 n + 4)."""
+    return 9 * n * (math.log2(n) + 4)
+
+
+def bytes_alg(n):
+    """Algorithmic HBM bytes per squaring of one candidate: 24 n R (R = 1 if the
+    state fits one CU, else 2 exchanges)."""
+    return 24 * n * (1 if n <= (1 << 13) else 2)
+
+
+def run_batched(ct, steps, warmup):
+    from ctwin_amd.exponent import exponent
+    ln, n, lanes = 14, 1 << 14, 256
+    rng = random.Random(SEED)
+    bs = [rng.randrange(1 << 18, 1 << 19) for _ in range(lanes)]
+    E = [exponent(b, n) for b in bs]
+    i0 = max(e.bit_length() for e in E) - 1
+    t = ct.Transform(ln, lanes, ct.KIND_RESIDENT)
+    t.set_b(bs); t.set_exponents(E); t.set(1)
+    t.run(i0, i0 - warmup + 1)
+    t.timer_start()
+    t.run(i0 - warmup, i0 - warmup - steps + 1)
+    ms = t.timer_stop()
+    t.close()
+    return lanes * steps / (ms / 1e3), ms, i0, bs
+
+
+def run_plain(ct, ln, b, steps, warmup):
+    from ctwin_amd.exponent import window_value
+    n = 1 << ln
+    Ew, i0 = window_value(b, n, steps + warmup + 8)
+    t = ct.Transform(ln, 1, ct.KIND_PLAIN)
+    t.set_b([b]); t.set_exponents([Ew]); t.set(1)
+    t.run(i0, i0 - warmup + 1)
+    t.timer_start()
+    t.run(i0 - warmup, i0 - warmup - steps + 1)
+    ms = t.timer_stop()
+    t.close()
+    return steps / (ms / 1e3), ms
+
+
+def cpu_baseline(budget_s=12.0):
+    """The oracle (plain-C restatement, one core) on a bounded sample of the
+    same workload: squarings of one n = 2^14 candidate from the batch."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as O
+    rng = random.Random(SEED)
+    b = rng.randrange(1 << 18, 1 << 19)
+    n = 1 << 14
+    e = O.Engine(n, b, 16)
+    d = np.array([random.Random(1).randrange(-b + 1, b) for _ in range(n)], np.int32)
+    count = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        e.square_dup(d, count & 1)
+        count += 1
+    dt = time.perf_counter() - t0
+    return {"value": count / dt, "unit": "squarings/s", "cores": 1, "kind": "port",
+            "sample": f"{count} squarings of one n=2^14 candidate (b={b}) by the oracle in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--big-steps", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the candidates shard one queue per GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import ctwin_amd as ct
+
+    sq_s, ms, i0, bs = run_batched(ct, args.steps, args.warmup)
+    print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps)", flush=True)
+    plain_sq_s, plain_ms = run_plain(ct, 14, bs[0], args.plain_steps, 5)
+    print(f"[bench] plain n=2^14 x1: {plain_sq_s:.4g} squarings/s", flush=True)
+    big_b = 65536 + 2 * random.Random(SEED).randrange(1 << 14)
+    big_sq_s, big_ms = run_plain(ct, 21, big_b, args.big_steps, 5)
+    print(f"[bench] plain n=2^21 x1: {big_sq_s:.4g} squarings/s", flush=True)
+
+    n = 1 << 14
+    per_cand = i0 + 2 * int(math.isqrt(i0)) + 64       # PRP loop + Gerbicz-Li tail (approx.)
+    valu_achieved = mul32_alg(n) * sq_s / 1e12
+    big_n = 1 << 21
+    hbm_achieved = bytes_alg(big_n) * big_sq_s / 1e9
+    out = {
+        "metric": "Fermat squarings/s (batched, n=2^14 digits, bit-exact)",
+        "value": sq_s,
+        "unit": "squarings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded b)",
+        "config": {"workload": "batched: 256 candidates b^(2^14) - b^(2^13) + 1, b in [2^18, 2^19), "
+                               "base-2 PRP loop, one persistent resident launch",
+                   "log2_n": 14, "candidates": 256, "seed": SEED},
+        "candidates_per_hour": 3600.0 * sq_s / per_cand,
+        "plain": {"squarings_per_s": plain_sq_s, "candidates_per_hour": 3600.0 * plain_sq_s / per_cand,
+                  "engine": "plain multipass, one candidate per launch sequence"},
+        "batched_vs_plain": sq_s / plain_sq_s,
+        "single_large": {"log2_n": 21, "b": big_b, "squarings_per_s": big_sq_s, "ms_per_squaring": big_ms / args.big_steps,
+                         "roofline": {"bound": "hbm", "achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": hbm_achieved / HBM_PEAK_GBS, "traffic": None,
+                                      "bytes_alg_per_squaring": bytes_alg(big_n)}},
+        "roofline": {"bound": "valu", "achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "Tmul32/s",
+                     "frac": valu_achieved / VALU_PEAK_T, "traffic": None,
+                     "kernel": "k_res_steps<14>", "mul32_alg_per_squaring": mul32_alg(n)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

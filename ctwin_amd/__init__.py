@@ -1,0 +1,220 @@
+"""ctwin_amd: MI355X-native engines for galloty/ctwin's data-parallel paths.
+
+Python mirror of the reference's host interfaces over the C ABI of
+include/ctwin.h (libctwin_amd.so, built in-tree by ctwin_amd/build.py):
+
+* Transform   -- cyclo/src/transform.h + engine.h: set_b, set, copy,
+                 square_dup, mul, get_int, is_prime, context state.
+* prp_check   -- cyclo/src/cyclo.h:252-411, base-2 Fermat PRP with Gerbicz-Li.
+* sieve_window -- sieve/CPU/ctsieve.cpp Sieve::check_pos/check_neg.
+
+The library is required: importing the engines without it raises, there is
+no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libctwin_amd.so")
+
+KIND_AUTO, KIND_RESIDENT, KIND_PLAIN = 0, 1, 2
+
+STATUS = {0: "CT_OK", -1: "CT_ERR_ARG", -2: "CT_ERR_HIP", -3: "CT_ERR_NOMEM", -4: "CT_ERR_UNSUPPORTED",
+          -5: "CT_ERR_GETINT", -6: "CT_ERR_COMPUTE", -7: "CT_ERR_GERBICZ"}
+
+EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engine_set", "ct_engine_copy",
+           "ct_engine_square_dup", "ct_engine_mul", "ct_engine_mul_regs", "ct_engine_get_int",
+           "ct_engine_is_prime", "ct_engine_read_state", "ct_engine_write_state", "ct_engine_set_exponents",
+           "ct_engine_run", "ct_prp_check", "ct_engine_timer_start", "ct_engine_timer_stop", "ct_engine_info",
+           "ct_sieve_window", "ct_last_error"]
+
+
+class CtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libctwin_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m ctwin_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        c = ctypes
+        vp, u32, i32, u64 = c.c_void_p, c.c_uint32, c.c_int, c.c_uint64
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        sig = {
+            "ct_engine_create": [i32, u32, u32, u32, i32, c.POINTER(vp)],
+            "ct_engine_destroy": [vp],
+            "ct_engine_set_b": [vp, u32p],
+            "ct_engine_set": [vp, u32],
+            "ct_engine_copy": [vp, u32, u32],
+            "ct_engine_square_dup": [vp, c.c_void_p],
+            "ct_engine_mul": [vp],
+            "ct_engine_mul_regs": [vp, u32, u32, u32],
+            "ct_engine_get_int": [vp, u32, u32p],
+            "ct_engine_is_prime": [vp, u32p, ip, u64p],
+            "ct_engine_read_state": [vp, u32, i32p],
+            "ct_engine_write_state": [vp, u32, i32p],
+            "ct_engine_set_exponents": [vp, u32p, u32],
+            "ct_engine_run": [vp, i32, i32, i32],
+            "ct_prp_check": [vp, u32p, ip, u64p, ip, c.c_void_p],
+            "ct_engine_timer_start": [vp],
+            "ct_engine_timer_stop": [vp, c.POINTER(c.c_float)],
+            "ct_engine_info": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(i32)],
+            "ct_sieve_window": [i32, i32, u64, u64, u64, u64, u64, u64, i32, u8p],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = c.c_int
+        L.ct_last_error.argtypes = []
+        L.ct_last_error.restype = c.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise CtError(rc, lib().ct_last_error().decode())
+    return rc
+
+
+class Transform:
+    """transform + engine of cyclo/src: `lanes` candidates of n = 2^log2_n digits."""
+
+    def __init__(self, log2_n, lanes=1, kind=KIND_AUTO, device=0, csize=16):
+        self.n = 1 << log2_n
+        self.log2_n = log2_n
+        self.lanes = lanes
+        h = ctypes.c_void_p()
+        _check(lib().ct_engine_create(device, log2_n, lanes, csize, kind, ctypes.byref(h)))
+        self.h = h
+        n_ = ctypes.c_uint32(); l_ = ctypes.c_uint32(); k_ = ctypes.c_int()
+        _check(lib().ct_engine_info(self.h, ctypes.byref(n_), ctypes.byref(l_), ctypes.byref(k_)))
+        self.kind = k_.value
+        self.b = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ct_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_b(self, b):
+        b = np.ascontiguousarray(b, np.uint32)
+        assert b.shape == (self.lanes,)
+        _check(lib().ct_engine_set_b(self.h, b))
+        self.b = b.copy()
+
+    def set(self, a):
+        _check(lib().ct_engine_set(self.h, int(a)))
+
+    def copy(self, dst, src):
+        _check(lib().ct_engine_copy(self.h, dst, src))
+
+    def square_dup(self, dup=None):
+        """dup: per-lane 0/1 sequence (or None)."""
+        if dup is None:
+            _check(lib().ct_engine_square_dup(self.h, None))
+            return
+        words = np.zeros((self.lanes + 63) // 64, np.uint64)
+        for l, bit in enumerate(dup):
+            if bit:
+                words[l // 64] |= np.uint64(1) << np.uint64(l % 64)
+        _check(lib().ct_engine_square_dup(self.h, words.ctypes.data_as(ctypes.c_void_p)))
+
+    def mul(self, dst=0, a=0, b=1):
+        _check(lib().ct_engine_mul_regs(self.h, dst, a, b))
+
+    def get_int(self, reg=0):
+        out = np.zeros(self.lanes * self.n, np.uint32)
+        _check(lib().ct_engine_get_int(self.h, reg, out))
+        return out.reshape(self.lanes, self.n)
+
+    def is_prime(self, canon):
+        canon = np.ascontiguousarray(canon, np.uint32).reshape(-1)
+        prm = np.zeros(self.lanes, np.int32); r64 = np.zeros(self.lanes, np.uint64)
+        rc = lib().ct_engine_is_prime(self.h, canon, prm, r64)
+        return rc != 0, prm.astype(bool), r64
+
+    def read_state(self, reg=0):
+        out = np.zeros(self.lanes * self.n, np.int32)
+        _check(lib().ct_engine_read_state(self.h, reg, out))
+        return out.reshape(self.lanes, self.n)
+
+    def write_state(self, reg, digits):
+        d = np.ascontiguousarray(digits, np.int32).reshape(-1)
+        _check(lib().ct_engine_write_state(self.h, reg, d))
+
+    def set_exponents(self, values):
+        """values: one Python int per lane (bit i of lane l drives squaring i)."""
+        top = max(int(v).bit_length() for v in values)
+        words = max(1, (top + 31) // 32)
+        bits = np.zeros((self.lanes, words), np.uint32)
+        for l, v in enumerate(values):
+            raw = int(v).to_bytes(4 * words, "little")
+            bits[l] = np.frombuffer(raw, np.uint32)
+        _check(lib().ct_engine_set_exponents(self.h, np.ascontiguousarray(bits.reshape(-1)), words))
+
+    def run(self, i_hi, i_lo=0, gl_block=0):
+        _check(lib().ct_engine_run(self.h, int(i_hi), int(i_lo), int(gl_block)))
+
+    def timer_start(self):
+        _check(lib().ct_engine_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = ctypes.c_float()
+        _check(lib().ct_engine_timer_stop(self.h, ctypes.byref(ms)))
+        return ms.value
+
+
+def prp_check(t, bs, want_residue=False):
+    """cyclo::check for one batch: returns (prm, res64, gl_ok, residue or None, status)."""
+    bs = np.ascontiguousarray(bs, np.uint32)
+    prm = np.zeros(t.lanes, np.int32); r64 = np.zeros(t.lanes, np.uint64); gl = np.zeros(t.lanes, np.int32)
+    res = np.zeros(t.lanes * t.n, np.uint32) if want_residue else None
+    rc = lib().ct_prp_check(t.h, bs, prm, r64, gl, res.ctypes.data_as(ctypes.c_void_p) if res is not None else None)
+    if rc not in (0, -6, -7):
+        _check(rc)
+    t.b = bs.copy()
+    return prm.astype(bool), r64, gl.astype(bool), (res.reshape(t.lanes, t.n) if res is not None else None), rc
+
+
+def res64_string(r):
+    """cyclo.h:51-57: uppercase 16-digit hex."""
+    return "%016X" % int(r)
+
+
+def sieve_window(m, b_min, b_max, p_max_pos, p_max_neg, mode=0, p_min_pos=None, p_min_neg=11, device=0,
+                 struck=None):
+    """Sieve::check(ini) over [b_min, b_max] with explicit p bounds; returns the struck bytes."""
+    if p_min_pos is None:
+        p_min_pos = 3 * (1 << m) + 1
+    if struck is None:
+        struck = np.zeros(b_max - b_min + 1, np.uint8)
+    _check(lib().ct_sieve_window(device, m, b_min, b_max, p_min_pos, p_max_pos, p_min_neg, p_max_neg, mode, struck))
+    return struck
+
+
+def survivors(struck, b_min):
+    return np.nonzero(struck == 0)[0].astype(np.uint64) + np.uint64(b_min)

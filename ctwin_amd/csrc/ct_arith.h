@@ -1,0 +1,193 @@
+// Exact modular and multi-word integer arithmetic shared by the gfx950 kernels
+// and the host code of the squaring engine.
+//
+// Residues live modulo three primes p < 2^31 with 3 * 2^23 | p - 1, so that the
+// negacyclic-like transform modulo X^n - X^(n/2) + 1 exists for n <= 2^23 and the
+// "min trick" reduces a sum or difference in three VALU instructions
+// (a + b < 2^32 because 2p < 2^32).  Products use Montgomery reduction with
+// R = 2^32 (P. L. Montgomery, Math. Comp. 44 (1985) 519-521); a multiply by a
+// constant w uses the precomputed wq = w * p^-1 mod 2^32 so that it costs
+// mul_lo + two mul_hi (all three are full-rate VALU on gfx950).
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define CT_HD __host__ __device__ __forceinline__
+#else
+#define CT_HD inline
+#endif
+
+namespace ct {
+
+// ---- prime set ------------------------------------------------------------
+
+constexpr int kPrimes = 3;
+constexpr uint32_t kP[kPrimes] = {2113929217u, 2088763393u, 2013265921u};   // 63*2^25+1, 249*2^23+1, 15*2^27+1
+constexpr uint32_t kG[kPrimes] = {5u, 5u, 31u};                              // primitive roots
+constexpr int kMaxLog2N = 23;                                                 // 3*2^23 | p-1 for all three
+
+constexpr uint32_t inv32(uint32_t p)   // p^-1 mod 2^32 by Newton iteration
+{
+	uint32_t x = p;                      // correct to 3 bits for odd p
+	for (int i = 0; i < 5; ++i) x *= 2u - p * x;
+	return x;
+}
+
+constexpr uint32_t kQ[kPrimes] = {inv32(kP[0]), inv32(kP[1]), inv32(kP[2])};
+static_assert(kQ[0] * kP[0] == 1u && kQ[1] * kP[1] == 1u && kQ[2] * kP[2] == 1u, "Montgomery inverse");
+static_assert(kP[0] < 0x80000000u && kP[1] < 0x80000000u && kP[2] < 0x80000000u, "min trick needs p < 2^31");
+
+// ---- mod p ----------------------------------------------------------------
+
+CT_HD uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+CT_HD uint32_t add_mod(uint32_t a, uint32_t b, uint32_t p) { const uint32_t s = a + b; return umin32(s, s - p); }
+CT_HD uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) { const uint32_t d = a - b; return umin32(d, d + p); }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+CT_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
+#else
+CT_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+#endif
+
+// Montgomery product a * b * 2^-32 mod p, a, b in [0, p).
+CT_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t q)
+{
+	const uint64_t t = (uint64_t)a * b;
+	const uint32_t m = (uint32_t)t * q;
+	const uint32_t r = (uint32_t)(t >> 32) - mulhi32(m, p);
+	return umin32(r, r + p);
+}
+
+// Montgomery product by a constant w with wq = w * q mod 2^32 precomputed.
+CT_HD uint32_t mont_mulc(uint32_t a, uint32_t w, uint32_t wq, uint32_t p)
+{
+	const uint32_t m = a * wq;
+	const uint32_t r = mulhi32(a, w) - mulhi32(m, p);
+	return umin32(r, r + p);
+}
+
+// Signed 32-bit digit -> residue.
+CT_HD uint32_t from_signed(int32_t d, uint32_t p) { return (d < 0) ? (uint32_t)((int64_t)d + p) : (uint32_t)d; }
+
+// ---- host-side (or constexpr) plain modular helpers -------------------------
+
+CT_HD uint32_t mulmod_plain(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)(((uint64_t)a * b) % p); }
+
+CT_HD uint32_t powmod_plain(uint32_t a, uint64_t e, uint32_t p)
+{
+	uint64_t r = 1, y = a % p;
+	while (e) { if (e & 1) r = (r * y) % p; y = (y * y) % p; e >>= 1; }
+	return (uint32_t)r;
+}
+
+CT_HD uint32_t to_mont(uint32_t a, uint32_t p) { return (uint32_t)((((uint64_t)a) << 32) % p); }
+
+// ---- signed 96-bit accumulator ----------------------------------------------
+
+struct i96 { uint64_t lo; int32_t hi; };
+
+CT_HD i96 i96_add(i96 x, i96 y)
+{
+	i96 r; r.lo = x.lo + y.lo;
+	r.hi = x.hi + y.hi + ((r.lo < y.lo) ? 1 : 0);
+	return r;
+}
+
+CT_HD i96 i96_from_i64(int64_t v) { i96 r; r.lo = (uint64_t)v; r.hi = (v < 0) ? -1 : 0; return r; }
+
+CT_HD bool i96_neg(i96 x) { return x.hi < 0; }
+
+CT_HD i96 i96_negate(i96 x)
+{
+	i96 r; r.lo = ~x.lo + 1;
+	r.hi = ~x.hi + ((r.lo == 0) ? 1 : 0);
+	return r;
+}
+
+// ---- exact division by the base b ---------------------------------------
+//
+// Division of a two-limb value (x1 < d) by a normalised divisor
+// d in [2^31, 2^32) with the precomputed reciprocal v = floor((2^64-1)/d) - 2^32
+// (N. Moller and T. Granlund, "Improved division by invariant integers",
+// IEEE Trans. Computers 60 (2011), algorithm 4).  Quotient < 2^32.
+
+struct divisor { uint32_t d; uint32_t v; uint32_t sh; uint32_t b; };
+
+CT_HD divisor make_divisor(uint32_t b)
+{
+	divisor r; r.b = b;
+	uint32_t sh = 0; while (((b << sh) & 0x80000000u) == 0) ++sh;
+	r.sh = sh; r.d = b << sh;
+	r.v = (uint32_t)((~(uint64_t)0) / r.d - ((uint64_t)1 << 32));
+	return r;
+}
+
+CT_HD uint32_t div_2by1(uint32_t x1, uint32_t x0, const divisor &dv, uint32_t *rem)
+{
+	const uint64_t qq = (uint64_t)dv.v * x1 + (((uint64_t)x1 << 32) | x0);
+	uint32_t q1 = (uint32_t)(qq >> 32) + 1;
+	const uint32_t q0 = (uint32_t)qq;
+	uint32_t r = x0 - q1 * dv.d;
+	if (r > q0) { q1 -= 1; r += dv.d; }
+	if (r >= dv.d) { q1 += 1; r -= dv.d; }
+	*rem = r;
+	return q1;
+}
+
+// |t| / b with t unsigned 96-bit (t2, t1, t0) and t < 2^64 * b:
+// returns the 64-bit quotient and the remainder.
+CT_HD uint64_t udiv96(uint32_t t2, uint32_t t1, uint32_t t0, const divisor &dv, uint32_t *rem)
+{
+	const uint32_t sh = dv.sh;
+	// normalise: shift the dividend left by sh (t2 < b ensures no overflow)
+	const uint32_t n2 = sh ? ((t2 << sh) | (t1 >> (32 - sh))) : t2;
+	const uint32_t n1 = sh ? ((t1 << sh) | (t0 >> (32 - sh))) : t1;
+	const uint32_t n0 = t0 << sh;
+	uint32_t r;
+	const uint32_t qh = div_2by1(n2, n1, dv, &r);
+	const uint32_t ql = div_2by1(r, n0, dv, &r);
+	*rem = r >> sh;
+	return ((uint64_t)qh << 32) | ql;
+}
+
+// Signed division with truncation toward zero (C semantics), as the carry
+// steps require: a = q * b + r with |r| < b and r of the sign of a.
+CT_HD int32_t divrem_i96(i96 &a, const divisor &dv)
+{
+	const bool s = i96_neg(a);
+	const i96 t = s ? i96_negate(a) : a;
+	uint32_t r;
+	const uint64_t q = udiv96((uint32_t)t.hi, (uint32_t)(t.lo >> 32), (uint32_t)t.lo, dv, &r);
+	a = i96_from_i64(s ? -(int64_t)q : (int64_t)q);
+	return s ? -(int32_t)r : (int32_t)r;
+}
+
+CT_HD int32_t divrem_i64(int64_t &a, const divisor &dv)
+{
+	const bool s = (a < 0);
+	const uint64_t t = s ? (uint64_t)(-a) : (uint64_t)a;
+	uint32_t r;
+	const uint64_t q = udiv96(0, (uint32_t)(t >> 32), (uint32_t)t, dv, &r);
+	a = s ? -(int64_t)q : (int64_t)q;
+	return s ? -(int32_t)r : (int32_t)r;
+}
+
+// ---- Garner CRT constants for the engine primes --------------------------
+//
+// x = r3 + p3 * (u23 + p2 * u123) with
+// u23 = (r2 - r3) / p3 mod p2, u13 = (r1 - r3) / p3 mod p1,
+// u123 = (u13 - u23) / p2 mod p1; centred in (-M/2, M/2].
+
+struct garner_consts
+{
+	uint32_t inv3_1, inv3_1q;   // Montgomery form of p3^-1 mod p1, and its q-product
+	uint32_t inv3_2, inv3_2q;   // p3^-1 mod p2
+	uint32_t inv2_1, inv2_1q;   // p2^-1 mod p1
+	uint32_t p3_mod_p1, p3_mod_p2;
+	uint32_t m_lo, m_mid, m_hi;        // M = p1 p2 p3 (96-bit)
+	uint32_t h_lo, h_mid, h_hi;        // floor(M / 2)
+};
+
+}   // namespace ct

@@ -1,0 +1,391 @@
+// Host side of the squaring engine: device buffers, kernel sequencing and the
+// C ABI of include/ctwin.h (the drop-in for cyclo/src/engine.h and the
+// engine-facing part of cyclo/src/transform.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ctwin.h"
+#include "ct_arith.h"
+#include "ct_tables.h"
+#include "ct_kernels.h"
+#include "ct_engine_impl.h"
+
+using namespace ct;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_err;
+
+const char *ct_last_error(void) { return g_err.c_str(); }
+
+int ct::fail(int code, const char *what)
+{
+	g_err = what;
+	return code;
+}
+
+int ct::hip_fail(hipError_t e, const char *where)
+{
+	g_err = std::string(where) + ": " + hipGetErrorString(e);
+	return CT_ERR_HIP;
+}
+
+#define HIPCHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) return hip_fail(_e, #call); } while (0)
+
+// ---------------------------------------------------------------------------
+// engine lifetime
+
+int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out)
+{
+	if (!out) return fail(CT_ERR_ARG, "out is null");
+	*out = nullptr;
+	if (log2_n < 5 || log2_n > (uint32_t)kMaxLog2N) return fail(CT_ERR_UNSUPPORTED, "log2_n must be in [5, 23]");
+	if (lanes == 0) return fail(CT_ERR_ARG, "lanes must be > 0");
+	if (csize != 0 && csize != 16) return fail(CT_ERR_UNSUPPORTED, "csize must be 0 or 16");
+	if (kind == CT_KIND_AUTO) kind = (log2_n <= 14) ? CT_KIND_RESIDENT : CT_KIND_PLAIN;
+	if (kind == CT_KIND_RESIDENT && log2_n > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need log2_n <= 14");
+	if (kind != CT_KIND_RESIDENT && kind != CT_KIND_PLAIN) return fail(CT_ERR_ARG, "unknown engine kind");
+	int ndev = 0;
+	HIPCHK(hipGetDeviceCount(&ndev));
+	if (device < 0 || device >= ndev) return fail(CT_ERR_HIP, "no such HIP device");
+	HIPCHK(hipSetDevice(device));
+
+	ct_engine *e = new ct_engine();
+	e->device = device; e->ln = log2_n; e->n = 1u << log2_n; e->lanes = lanes; e->csize = 16; e->kind = kind;
+	e->gc = make_garner();
+	e->b.assign(lanes, 0);
+	const size_t reg_words = (size_t)lanes * e->n;
+	int rc = CT_OK;
+	auto alloc = [&](void **p, size_t bytes) {
+		if (rc != CT_OK) return;
+		hipError_t h = hipMalloc(p, bytes);
+		if (h != hipSuccess) rc = fail(CT_ERR_NOMEM, "hipMalloc failed");
+	};
+	alloc((void **)&e->d_x, 3 * reg_words * sizeof(int32_t));
+	alloc((void **)&e->d_tw, 6 * (size_t)e->n * sizeof(tw_entry));
+	alloc((void **)&e->d_dv, lanes * sizeof(divisor));
+	alloc((void **)&e->d_mask, ((lanes + 31) / 32) * sizeof(uint32_t));
+	if (rc == CT_OK && kind == CT_KIND_PLAIN) rc = plain_alloc(e);
+	if (rc != CT_OK) { ct_engine_destroy(e); return rc; }
+	hipError_t h = hipStreamCreate(&e->st);
+	if (h == hipSuccess) h = hipEventCreate(&e->ev0);
+	if (h == hipSuccess) h = hipEventCreate(&e->ev1);
+	if (h != hipSuccess) { ct_engine_destroy(e); return hip_fail(h, "stream/event create"); }
+	const std::vector<tw_entry> tw = build_tables(e->n);
+	h = hipMemcpy(e->d_tw, tw.data(), tw.size() * sizeof(tw_entry), hipMemcpyHostToDevice);
+	if (h == hipSuccess) h = hipMemset(e->d_x, 0, 3 * reg_words * sizeof(int32_t));
+	if (h != hipSuccess) { ct_engine_destroy(e); return hip_fail(h, "table upload"); }
+	*out = e;
+	return CT_OK;
+}
+
+int ct_engine_destroy(ct_engine *e)
+{
+	if (!e) return CT_OK;
+	hipSetDevice(e->device);
+	if (e->st) hipStreamSynchronize(e->st);
+	hipFree(e->d_x); hipFree(e->d_tw); hipFree(e->d_dv); hipFree(e->d_mask); hipFree(e->d_bits);
+	plain_free(e);
+	if (e->ev0) hipEventDestroy(e->ev0);
+	if (e->ev1) hipEventDestroy(e->ev1);
+	if (e->st) hipStreamDestroy(e->st);
+	delete e;
+	return CT_OK;
+}
+
+int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (n) *n = e->n;
+	if (lanes) *lanes = e->lanes;
+	if (kind) *kind = e->kind;
+	return CT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// parameters and registers
+
+int ct_engine_set_b(ct_engine *e, const uint32_t *b)
+{
+	if (!e || !b) return fail(CT_ERR_ARG, "null argument");
+	std::vector<divisor> dv(e->lanes);
+	for (uint32_t l = 0; l < e->lanes; ++l)
+	{
+		if (b[l] <= 32 || b[l] >= 0x80000000u) return fail(CT_ERR_ARG, "b must satisfy 32 < b < 2^31");
+		e->b[l] = b[l];
+		dv[l] = make_divisor(b[l]);
+	}
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipStreamSynchronize(e->st));
+	HIPCHK(hipMemcpy(e->d_dv, dv.data(), dv.size() * sizeof(divisor), hipMemcpyHostToDevice));
+	e->b_set = true;
+	return CT_OK;
+}
+
+int ct_engine_set(ct_engine *e, uint32_t a)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(launch_set(e->d_x, e->n, e->lanes, (int32_t)a, e->st));
+	e->pending = false;
+	return CT_OK;
+}
+
+// Brings register 0 to its carried form before any operation other than a
+// squaring reads it (the plain engine defers the last carry step).
+static int settle(ct_engine *e)
+{
+	if (e->kind == CT_KIND_PLAIN && e->pending) return plain_materialize(e);
+	return CT_OK;
+}
+
+int ct_engine_copy(ct_engine *e, uint32_t dst, uint32_t src)
+{
+	if (!e || dst > 2 || src > 2) return fail(CT_ERR_ARG, "register must be 0, 1 or 2");
+	if (dst == src) return CT_OK;
+	HIPCHK(hipSetDevice(e->device));
+	int rc = settle(e);
+	if (rc) return rc;
+	HIPCHK(hipMemcpyAsync(e->reg(dst), e->reg(src), e->reg_bytes(), hipMemcpyDeviceToDevice, e->st));
+	return CT_OK;
+}
+
+int ct_engine_read_state(ct_engine *e, uint32_t reg, int32_t *digits)
+{
+	if (!e || !digits || reg > 2) return fail(CT_ERR_ARG, "bad argument");
+	HIPCHK(hipSetDevice(e->device));
+	int rc = settle(e);
+	if (rc) return rc;
+	HIPCHK(hipStreamSynchronize(e->st));
+	HIPCHK(hipMemcpy(digits, e->reg(reg), e->reg_bytes(), hipMemcpyDeviceToHost));
+	return CT_OK;
+}
+
+int ct_engine_write_state(ct_engine *e, uint32_t reg, const int32_t *digits)
+{
+	if (!e || !digits || reg > 2) return fail(CT_ERR_ARG, "bad argument");
+	HIPCHK(hipSetDevice(e->device));
+	int rc = settle(e);
+	if (rc) return rc;
+	HIPCHK(hipStreamSynchronize(e->st));
+	HIPCHK(hipMemcpy(e->reg(reg), digits, e->reg_bytes(), hipMemcpyHostToDevice));
+	return CT_OK;
+}
+
+int ct_engine_set_exponents(ct_engine *e, const uint32_t *bits, uint32_t words)
+{
+	if (!e || !bits || words == 0) return fail(CT_ERR_ARG, "bad argument");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipStreamSynchronize(e->st));
+	const size_t bytes = (size_t)e->lanes * words * sizeof(uint32_t);
+	if (words != e->bit_words)
+	{
+		hipFree(e->d_bits); e->d_bits = nullptr; e->bit_words = 0;
+		if (hipMalloc((void **)&e->d_bits, bytes) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc exponent bits");
+		e->bit_words = words;
+	}
+	HIPCHK(hipMemcpy(e->d_bits, bits, bytes, hipMemcpyHostToDevice));
+	return CT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// squarings and products
+
+static int check_ready(const ct_engine *e)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (!e->b_set) return fail(CT_ERR_ARG, "set_b has not been called");
+	return CT_OK;
+}
+
+int ct::steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words)
+{
+	if (nsteps <= 0) return CT_OK;
+	if (e->kind == CT_KIND_PLAIN) return plain_steps(e, bits, bit_words, i_hi, nsteps, mask_words);
+	res_args A;
+	A.x = e->reg(0); A.tw = e->d_tw; A.dv = e->d_dv;
+	A.bits = bits; A.bit_words = bit_words; A.mask = mask_words;
+	A.i_hi = i_hi; A.nsteps = nsteps; A.gc = e->gc;
+	HIPCHK(resident_steps((int)e->ln, A, e->lanes, e->st));
+	return CT_OK;
+}
+
+int ct::mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
+{
+	if (e->kind == CT_KIND_PLAIN) return plain_mul(e, dst, a, b);
+	mul_args A;
+	A.a = e->reg(a); A.b = e->reg(b); A.out = e->reg(dst); A.tw = e->d_tw; A.dv = e->d_dv; A.gc = e->gc;
+	HIPCHK(resident_mul((int)e->ln, A, e->lanes, e->st));
+	return CT_OK;
+}
+
+int ct_engine_square_dup(ct_engine *e, const uint64_t *dup)
+{
+	int rc = check_ready(e);
+	if (rc) return rc;
+	HIPCHK(hipSetDevice(e->device));
+	const uint32_t words = (e->lanes + 31) / 32;
+	const uint32_t *mask = nullptr;
+	if (dup)
+	{
+		std::vector<uint32_t> m(words, 0);
+		for (uint32_t l = 0; l < e->lanes; ++l)
+			if ((dup[l / 64] >> (l % 64)) & 1u) m[l / 32] |= 1u << (l % 32);
+		HIPCHK(hipStreamSynchronize(e->st));
+		HIPCHK(hipMemcpy(e->d_mask, m.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice));
+		mask = e->d_mask;
+	}
+	return steps(e, nullptr, 0, 0, 1, mask);
+}
+
+int ct_engine_mul(ct_engine *e) { return ct_engine_mul_regs(e, 0, 0, 1); }
+
+int ct_engine_mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
+{
+	int rc = check_ready(e);
+	if (rc) return rc;
+	if (dst > 2 || a > 2 || b > 2) return fail(CT_ERR_ARG, "register must be 0, 1 or 2");
+	HIPCHK(hipSetDevice(e->device));
+	return mul_regs(e, dst, a, b);
+}
+
+int ct_engine_run(ct_engine *e, int i_hi, int i_lo, int gl_block)
+{
+	int rc = check_ready(e);
+	if (rc) return rc;
+	if (!e->d_bits) return fail(CT_ERR_ARG, "set_exponents has not been called");
+	if (i_lo < 0 || i_hi < i_lo) return CT_OK;
+	if ((uint64_t)i_hi >= (uint64_t)e->bit_words * 32) return fail(CT_ERR_ARG, "i_hi beyond the exponent bits");
+	HIPCHK(hipSetDevice(e->device));
+	int i = i_hi;
+	while (i >= i_lo)
+	{
+		// next Gerbicz-Li point g <= i (i % B == 0, i / B != 0), cyclo.h:301
+		int stop = i_lo;
+		if (gl_block > 0)
+		{
+			const int g = (i / gl_block) * gl_block;
+			if (g >= i_lo && g / gl_block != 0) stop = g;
+		}
+		rc = steps(e, e->d_bits, e->bit_words, i, i - stop + 1, nullptr);
+		if (rc) return rc;
+		if (gl_block > 0 && stop % gl_block == 0 && stop / gl_block != 0)
+		{
+			rc = mul_regs(e, 1, 0, 1);
+			if (rc) return rc;
+		}
+		i = stop - 1;
+	}
+	return CT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// read-out
+
+// Canonical digits of one lane, the algorithm of transform.h:501-597:
+// floor-carry in base b, wrap X^n = X^(n/2) - 1 until no carry is left, then
+// subtract N once when the value is >= N.
+int ct::canonical(const int32_t *d, uint32_t n, uint32_t b, uint32_t *x)
+{
+	auto fl = [b](int64_t &f) -> uint32_t {
+		int64_t q = f / (int64_t)b, r = f % (int64_t)b;
+		if (r < 0) { r += b; --q; }
+		f = q;
+		return (uint32_t)r;
+	};
+	int64_t f = 0;
+	for (uint32_t j = 0; j < n; ++j) { int64_t v = f + d[j]; x[j] = fl(v); f = v; }
+	int guard = 0;
+	while (f != 0)
+	{
+		if (++guard > 64) return CT_ERR_GETINT;
+		int64_t v = -f;
+		for (uint32_t j = 0; j < n / 2; ++j) { v += (int64_t)x[j]; x[j] = fl(v); }
+		v += f;
+		for (uint32_t j = n / 2; j < n; ++j) { v += (int64_t)x[j]; x[j] = fl(v); if (v == 0) break; }
+		f = v;
+	}
+	bool over = true;
+	for (uint32_t j = n / 2; j < n; ++j) if (x[j] != b - 1) { over = false; break; }
+	if (over)
+	{
+		bool low = false;
+		for (uint32_t j = 0; j < n / 2; ++j) if (x[j] != 0) { low = true; break; }
+		over = low;
+	}
+	if (over)
+	{   // value - N = (low half) - 1, top half cleared
+		int32_t borrow = 1;
+		for (uint32_t j = 0; j < n / 2 && borrow; ++j)
+		{
+			if (x[j] >= (uint32_t)borrow) { x[j] -= borrow; borrow = 0; }
+			else { x[j] = b - 1; }
+		}
+		if (borrow) return CT_ERR_GETINT;
+		for (uint32_t j = n / 2; j < n; ++j) x[j] = 0;
+	}
+	return CT_OK;
+}
+
+int ct_engine_get_int(ct_engine *e, uint32_t reg, uint32_t *digits)
+{
+	if (!e || !digits || reg > 2) return fail(CT_ERR_ARG, "bad argument");
+	if (!e->b_set) return fail(CT_ERR_ARG, "set_b has not been called");
+	std::vector<int32_t> raw((size_t)e->lanes * e->n);
+	int rc = ct_engine_read_state(e, reg, raw.data());
+	if (rc) return rc;
+	for (uint32_t l = 0; l < e->lanes; ++l)
+	{
+		rc = canonical(&raw[(size_t)l * e->n], e->n, e->b[l], &digits[(size_t)l * e->n]);
+		if (rc) return fail(CT_ERR_GETINT, "getInt failed");
+	}
+	return CT_OK;
+}
+
+int ct_engine_is_prime(const ct_engine *e, const uint32_t *digits, int *prm, uint64_t *res64)
+{
+	if (!e || !digits || !prm || !res64) return fail(CT_ERR_ARG, "bad argument");
+	bool err = false;
+	for (uint32_t l = 0; l < e->lanes; ++l)
+	{
+		const uint32_t *x = &digits[(size_t)l * e->n];
+		bool one = (x[0] == 1), zero = (x[0] == 0);
+		uint64_t r = x[0], bi = e->b[l];
+		for (uint32_t j = 1; j < e->n; ++j)
+		{
+			one &= (x[j] == 0); zero &= (x[j] == 0);
+			r += (uint64_t)x[j] * bi;
+			bi *= e->b[l];
+		}
+		prm[l] = one ? 1 : 0; res64[l] = r;
+		err |= zero;
+	}
+	return err ? fail(CT_ERR_COMPUTE, "computation failed") : CT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// timing
+
+int ct_engine_timer_start(ct_engine *e)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipEventRecord(e->ev0, e->st));
+	return CT_OK;
+}
+
+int ct_engine_timer_stop(ct_engine *e, float *ms)
+{
+	if (!e || !ms) return fail(CT_ERR_ARG, "bad argument");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipEventRecord(e->ev1, e->st));
+	HIPCHK(hipEventSynchronize(e->ev1));
+	HIPCHK(hipEventElapsedTime(ms, e->ev0, e->ev1));
+	return CT_OK;
+}

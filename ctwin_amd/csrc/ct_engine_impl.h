@@ -1,0 +1,55 @@
+// Internal engine state shared by the host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "ct_arith.h"
+#include "ct_tables.h"
+
+struct ct_engine
+{
+	int device = 0;
+	uint32_t ln = 0, n = 0, lanes = 0, csize = 16;
+	int kind = 0;
+	hipStream_t st = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	int32_t *d_x = nullptr;           // 3 registers x [lanes][n] signed digits
+	ct::tw_entry *d_tw = nullptr;     // [3][2][n]
+	ct::divisor *d_dv = nullptr;      // [lanes]
+	uint32_t *d_mask = nullptr;       // [lanes/32] dup bits of one squareDup
+	uint32_t *d_bits = nullptr;       // [lanes][bit_words] exponent bits
+	uint32_t bit_words = 0;
+	bool b_set = false;
+	std::vector<uint32_t> b;
+	ct::garner_consts gc{};
+	// multipass (plain) engine scratch
+	uint32_t *d_res = nullptr;        // [3][lanes][n] residues between passes
+	uint32_t *d_res2 = nullptr;       // second operand of a product
+	int64_t *d_carry = nullptr;       // [lanes][n/16] chunk carries
+	uint32_t rows_log2 = 0;           // column length R = 2^rows_log2
+	bool pending = false;             // register 0 still owes the chunk carries (plain engine)
+
+	int32_t *reg(uint32_t r) const { return d_x + (size_t)r * lanes * n; }
+	size_t reg_bytes() const { return (size_t)lanes * n * sizeof(int32_t); }
+};
+
+namespace ct {
+
+int fail(int code, const char *what);
+int hip_fail(hipError_t e, const char *where);
+int canonical(const int32_t *d, uint32_t n, uint32_t b, uint32_t *x);
+int steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words);
+int mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
+
+hipError_t launch_set(int32_t *x, uint32_t n, uint32_t lanes, int32_t a, hipStream_t st);
+
+int plain_alloc(ct_engine *e);
+void plain_free(ct_engine *e);
+int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words);
+int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
+int plain_materialize(ct_engine *e);
+
+}   // namespace ct

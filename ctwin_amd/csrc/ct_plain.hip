@@ -1,0 +1,393 @@
+// Plain multipass squaring engine: one squaring is three launches over the
+// whole GPU, for n = 2^LN digits with 10 <= LN <= 22 (one candidate per launch
+// in the "plain" configuration; any number of lanes share each launch).
+//
+// n = R x C (R = 2^RL rows, C = 2^CL columns, k = r C + c).  The first RL
+// levels of the DIF tree pair elements of one column; the remaining levels
+// stay inside one row.  So a squaring is
+//   A  column pass: load 16-digit chunks of W adjacent columns, apply the
+//      pending carries of the previous squaring (normalize2, kernel.cl:1053),
+//      then per prime the RL column levels (radix-16 passes, LDS transposes)
+//      -> three residue planes in HBM;
+//   B  row pass (one workgroup per row and prime): levels RL+1..LN-1, the
+//      pointwise square (sqr2) and the inverse row levels, in place;
+//   C  column pass: inverse column levels and the Phi_6 merge per prime,
+//      Garner CRT and normalize1 on 16-digit chunks -> digits + chunk carries.
+// HBM traffic per squaring and digit: A 4+0.5 read, 12 write; B 12 + 12;
+// C 12 read, 4+0.5 write = 57 bytes.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/ctwin.h"
+#include "ct_engine_impl.h"
+#include "ct_ntt.h"
+
+namespace ct {
+
+struct plain_args
+{
+	int32_t *x;                 // digits of the source / destination register
+	const int32_t *xb;          // second operand digits (product only)
+	uint32_t *res;              // [3][lanes][n] residue planes
+	uint32_t *res2;             // second operand planes (product only)
+	int64_t *carry;             // [lanes][n/16] chunk carries
+	const int64_t *carry_in;    // pending carries to apply on load (or null)
+	const tw_entry *tw;
+	const divisor *dv;
+	const uint32_t *bits;       // exponent bits or null
+	const uint32_t *mask;       // dup bits when bits is null (or null)
+	uint32_t bit_words;
+	int i;                      // exponent bit index of this squaring
+	uint32_t lanes;
+	garner_consts gc;
+};
+
+template <int RL>
+struct colgeo
+{
+	static constexpr uint32_t R = 1u << RL;
+	static constexpr uint32_t Tc = R / 16;                 // threads per column
+	static constexpr uint32_t W = (RL == 4) ? 64 : 16;      // columns per workgroup
+	static constexpr uint32_t threads = W * Tc;            // = chunks per workgroup (R W / 16)
+	static constexpr uint32_t RP = R + (R >> 5);           // padded column stride in LDS
+	static constexpr int passes = RL / 4;
+};
+
+// column forward passes 0..passes-1 (transposes between them only)
+template <int RL, int P, int L>
+__device__ __forceinline__ void col_fwd(uint32_t (&u)[E], uint32_t *X, const tw_entry *twf, uint32_t tt)
+{
+	fwd_pass<RL, P, L, true>(u, twf, 1u, tt);
+	if constexpr (L + 1 < colgeo<RL>::passes)
+	{
+		xchg<RL, L, L + 1>(u, X, tt);
+		col_fwd<RL, P, L + 1>(u, X, twf, tt);
+	}
+}
+
+template <int RL, int P, int L>
+__device__ __forceinline__ void col_inv(uint32_t (&u)[E], uint32_t *X, const tw_entry *twi, uint32_t tt)
+{
+	inv_pass<RL, P, L, true>(u, twi, 1u, tt);
+	if constexpr (L > 0)
+	{
+		xchg<RL, L, L - 1>(u, X, tt);
+		col_inv<RL, P, L - 1>(u, X, twi, tt);
+	}
+}
+
+template <int LN, int RL>
+__device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, uint32_t *res_lane, const tw_entry *tw,
+                                               uint32_t c0, uint32_t cw, uint32_t tt, size_t plane)
+{
+	using G = colgeo<RL>;
+	constexpr uint32_t n = 1u << LN, C = n >> RL;
+	constexpr int La = G::passes - 1;
+#pragma unroll
+	for (int P = 0; P < 3; ++P)
+	{
+		uint32_t u[E];
+		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
+		const uint32_t p = kP[P];
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = from_signed(dr[loff<RL, 0>(q)], p);
+		const tw_entry *twf = tw + (size_t)(P * 2) * n;
+		uint32_t *Xc = X + cw * G::RP;
+		if (P == 0) col_fwd<RL, 0, 0>(u, Xc, twf, tt);
+		else if (P == 1) col_fwd<RL, 1, 0>(u, Xc, twf, tt);
+		else col_fwd<RL, 2, 0>(u, Xc, twf, tt);
+		uint32_t *out = res_lane + P * plane + c0 + cw;
+#pragma unroll
+		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, La>(tt, q) * C] = u[q];
+	}
+}
+
+// A: chunk load (+ pending carry), column levels, residue planes
+template <int LN, int RL>
+__global__ __launch_bounds__(colgeo<RL>::threads) void k_colA(plain_args A, int operand)
+{
+	using G = colgeo<RL>;
+	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
+	__shared__ int32_t D[G::W * G::RP];
+	__shared__ uint32_t X[G::W * G::RP];
+	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
+	const int32_t *x = (operand ? A.xb : A.x) + (size_t)lane * n;
+	uint32_t *res = (operand ? A.res2 : A.res) + (size_t)lane * n;
+	// chunk role: row r, chunk h of the W columns
+	{
+		constexpr uint32_t hpr = G::W / 16;
+		const uint32_t r = t / hpr, h = t % hpr;
+		const uint32_t k = r * C + c0 + 16 * h;
+		int32_t dig[E];
+		const int4 *src = reinterpret_cast<const int4 *>(x + k);
+#pragma unroll
+		for (int v = 0; v < 4; ++v) { const int4 w = src[v]; dig[4 * v] = w.x; dig[4 * v + 1] = w.y; dig[4 * v + 2] = w.z; dig[4 * v + 3] = w.w; }
+		if (A.carry_in && operand == 0)
+		{
+			const uint32_t g = k / 16;
+			const int64_t *f = A.carry_in + (size_t)lane * nc;
+			int64_t c = f[(g + nc - 1) % nc];
+			if (g == 0) c = -c;
+			else if (g == nc / 2) c += f[nc - 1];
+			carry_in(dig, c, A.dv[lane]);
+		}
+#pragma unroll
+		for (int q = 0; q < E; ++q) D[(16 * h + q) * G::RP + r + (r >> 5)] = dig[q];
+	}
+	__syncthreads();
+	const uint32_t cw = t % G::W, tt = t / G::W;
+	colA_prime_all<LN, RL>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n);
+}
+
+// B: row levels with the pointwise square / product, in place
+template <int LN, int RL, int P, bool MUL>
+__device__ __forceinline__ void rowB_prime(plain_args &A, uint32_t *lds, uint32_t r, uint32_t lane, uint32_t t)
+{
+	constexpr int CL = LN - RL;
+	constexpr uint32_t n = 1u << LN, C = 1u << CL, T = C / 16, R = 1u << RL;
+	const size_t plane = (size_t)A.lanes * n;
+	uint32_t *row = A.res + P * plane + (size_t)lane * n + (size_t)r * C;
+	const tw_entry *twf = A.tw + (size_t)(P * 2) * n, *twi = A.tw + (size_t)(P * 2 + 1) * n;
+	const uint32_t base = R + r;
+	uint32_t u[E], v[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q) u[q] = row[t + q * T];
+	fwd_chain<CL, P, 0, false>(u, lds, twf, base, t);
+	if (MUL)
+	{
+		const uint32_t *row2 = A.res2 + P * plane + (size_t)lane * n + (size_t)r * C;
+#pragma unroll
+		for (int q = 0; q < E; ++q) v[q] = row2[t + q * T];
+		fwd_chain<CL, P, 0, false>(v, lds, twf, base, t);
+	}
+	else
+	{
+#pragma unroll
+		for (int q = 0; q < E; ++q) v[q] = 0;
+	}
+	final_pass<CL, P, MUL>(u, v, twf, twi, base, t);
+	inv_chain<CL, P, geo<CL>::a - 1, false>(u, lds, twi, base, t);
+#pragma unroll
+	for (int q = 0; q < E; ++q) row[t + q * T] = u[q];
+}
+
+template <int LN, int RL, bool MUL>
+__global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
+{
+	constexpr int CL = LN - RL;
+	__shared__ uint32_t lds[geo<CL>::lds_words];
+	const uint32_t r = blockIdx.x, P = blockIdx.y, lane = blockIdx.z, t = threadIdx.x;
+	if (P == 0) rowB_prime<LN, RL, 0, MUL>(A, lds, r, lane, t);
+	else if (P == 1) rowB_prime<LN, RL, 1, MUL>(A, lds, r, lane, t);
+	else rowB_prime<LN, RL, 2, MUL>(A, lds, r, lane, t);
+}
+
+// C: inverse column levels, CRT, normalize1
+template <int LN, int RL>
+__global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int no_dup)
+{
+	using G = colgeo<RL>;
+	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
+	constexpr int La = G::passes - 1;
+	__shared__ uint32_t S[3][G::W * G::RP];
+	__shared__ uint32_t X[G::W * G::RP];
+	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
+	const uint32_t cw = t % G::W, tt = t / G::W;
+	const size_t plane = (size_t)A.lanes * n;
+	const uint32_t *res = A.res + (size_t)lane * n;
+#pragma unroll
+	for (int P = 0; P < 3; ++P)
+	{
+		uint32_t u[E];
+		const uint32_t *in = res + P * plane + c0 + cw;
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = in[(size_t)elem<RL, La>(tt, q) * C];
+		const tw_entry *twi = A.tw + (size_t)(P * 2 + 1) * n;
+		uint32_t *Xc = X + cw * G::RP;
+		if (P == 0) col_inv<RL, 0, La>(u, Xc, twi, tt);
+		else if (P == 1) col_inv<RL, 1, La>(u, Xc, twi, tt);
+		else col_inv<RL, 2, La>(u, Xc, twi, tt);
+		uint32_t *sw = S[P] + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) sw[loff<RL, 0>(q)] = u[q];
+	}
+	__syncthreads();
+	// chunk role
+	constexpr uint32_t hpr = G::W / 16;
+	const uint32_t r = t / hpr, h = t % hpr;
+	const uint32_t k = r * C + c0 + 16 * h;
+	const uint32_t rr = r + (r >> 5);
+	i96 l[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q)
+	{
+		const uint32_t a = (16 * h + q) * G::RP + rr;
+		l[q] = garner(S[0][a], S[1][a], S[2][a], A.gc);
+	}
+	bool dup = false;
+	if (!no_dup)
+	{
+		if (A.bits) dup = (A.bits[(size_t)lane * A.bit_words + (A.i >> 5)] >> (A.i & 31)) & 1u;
+		else if (A.mask) dup = (A.mask[lane >> 5] >> (lane & 31)) & 1u;
+	}
+	int32_t dig[E];
+	const int64_t f = carry_chunk(l, dig, dup, A.dv[lane]);
+	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * n + k);
+#pragma unroll
+	for (int v = 0; v < 4; ++v) dst[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
+	A.carry[(size_t)lane * nc + k / 16] = f;
+}
+
+// normalize2 alone: materialises the pending carries into the digits
+__global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, uint32_t n, uint32_t lanes)
+{
+	const uint32_t nc = n / 16;
+	const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (id >= (size_t)nc * lanes) return;
+	const uint32_t lane = (uint32_t)(id / nc), g = (uint32_t)(id % nc);
+	const int64_t *f = carry + (size_t)lane * nc;
+	int64_t c = f[(g + nc - 1) % nc];
+	if (g == 0) c = -c;
+	else if (g == nc / 2) c += f[nc - 1];
+	int4 *p = reinterpret_cast<int4 *>(x + (size_t)lane * n + 16 * (size_t)g);
+	int32_t dig[E];
+#pragma unroll
+	for (int v = 0; v < 4; ++v) { const int4 w = p[v]; dig[4 * v] = w.x; dig[4 * v + 1] = w.y; dig[4 * v + 2] = w.z; dig[4 * v + 3] = w.w; }
+	carry_in(dig, c, dvs[lane]);
+#pragma unroll
+	for (int v = 0; v < 4; ++v) p[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
+}
+
+// ---------------------------------------------------------------------------
+// host sequencing
+
+static int rows_log2(int ln) { return (ln <= 12) ? 4 : 8; }
+
+template <int LN, int RL>
+static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
+{
+	using G = colgeo<RL>;
+	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
+	const dim3 gcol(C / G::W, lanes);
+	hipLaunchKernelGGL((k_colA<LN, RL>), gcol, dim3(G::threads), 0, st, A, 0);
+	hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, 3, lanes), dim3(C / 16), 0, st, A);
+	hipLaunchKernelGGL((k_colC<LN, RL>), gcol, dim3(G::threads), 0, st, A, 0);
+	return hipGetLastError();
+}
+
+static hipError_t square_any(int ln, const plain_args &A, uint32_t lanes, hipStream_t st)
+{
+	switch (ln)
+	{
+	case 10: return square_ln<10, 4>(A, lanes, st);
+	case 11: return square_ln<11, 4>(A, lanes, st);
+	case 12: return square_ln<12, 4>(A, lanes, st);
+	case 13: return square_ln<13, 8>(A, lanes, st);
+	case 14: return square_ln<14, 8>(A, lanes, st);
+	case 15: return square_ln<15, 8>(A, lanes, st);
+	case 16: return square_ln<16, 8>(A, lanes, st);
+	case 17: return square_ln<17, 8>(A, lanes, st);
+	case 18: return square_ln<18, 8>(A, lanes, st);
+	case 19: return square_ln<19, 8>(A, lanes, st);
+	case 20: return square_ln<20, 8>(A, lanes, st);
+	case 21: return square_ln<21, 8>(A, lanes, st);
+	case 22: return square_ln<22, 8>(A, lanes, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+#define HIPCHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) return hip_fail(_e, #call); } while (0)
+
+int plain_alloc(ct_engine *e)
+{
+	if (e->ln < 10 || e->ln > 22) return fail(CT_ERR_UNSUPPORTED, "plain engine needs 10 <= log2_n <= 22");
+	e->rows_log2 = (uint32_t)rows_log2((int)e->ln);
+	const size_t words = (size_t)3 * e->lanes * e->n;
+	if (hipMalloc((void **)&e->d_res, words * sizeof(uint32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc residues");
+	if (hipMalloc((void **)&e->d_res2, words * sizeof(uint32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc residues");
+	if (hipMalloc((void **)&e->d_carry, (size_t)e->lanes * (e->n / 16) * sizeof(int64_t)) != hipSuccess)
+		return fail(CT_ERR_NOMEM, "hipMalloc carries");
+	return CT_OK;
+}
+
+void plain_free(ct_engine *e)
+{
+	hipFree(e->d_res); hipFree(e->d_res2); hipFree(e->d_carry);
+	e->d_res = e->d_res2 = nullptr; e->d_carry = nullptr;
+}
+
+static plain_args make_args(ct_engine *e)
+{
+	plain_args A;
+	std::memset(&A, 0, sizeof(A));
+	A.res = e->d_res; A.res2 = e->d_res2; A.carry = e->d_carry;
+	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc;
+	return A;
+}
+
+// Applies the pending carries of register 0 (after squarings).
+int plain_materialize(ct_engine *e)
+{
+	if (!e->pending) return CT_OK;
+	const size_t total = (size_t)e->lanes * (e->n / 16);
+	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,
+	                   e->reg(0), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes);
+	HIPCHK(hipGetLastError());
+	e->pending = false;
+	return CT_OK;
+}
+
+int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words)
+{
+	plain_args A = make_args(e);
+	A.x = e->reg(0); A.bits = bits; A.bit_words = bit_words; A.mask = mask_words;
+	for (int s = 0; s < nsteps; ++s)
+	{
+		A.i = i_hi - s;
+		A.carry_in = e->pending ? e->d_carry : nullptr;
+		HIPCHK(square_any((int)e->ln, A, e->lanes, e->st));
+		e->pending = true;
+	}
+	return CT_OK;
+}
+
+int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
+{
+	int rc = plain_materialize(e);
+	if (rc) return rc;
+	// In: the two operands (column pass A reads x and xb); Out: dst (row pass
+	// B and column pass C).  dst may alias a or b: both are read into the
+	// residue planes before C writes dst.
+	plain_args In = make_args(e);
+	In.x = e->reg(a); In.xb = e->reg(b);
+	plain_args Out = make_args(e);
+	Out.x = e->reg(dst);
+	hipError_t h = hipSuccess;
+	switch ((int)e->ln)
+	{
+#define CT_MUL_CASE(L_, R_) case L_: { \
+		using G = colgeo<R_>; constexpr uint32_t C = 1u << (L_ - R_), R = 1u << R_; \
+		const dim3 gcol(C / G::W, e->lanes); \
+		hipLaunchKernelGGL((k_colA<L_, R_>), gcol, dim3(G::threads), 0, e->st, In, 0); \
+		hipLaunchKernelGGL((k_colA<L_, R_>), gcol, dim3(G::threads), 0, e->st, In, 1); \
+		hipLaunchKernelGGL((k_rowB<L_, R_, true>), dim3(R, 3, e->lanes), dim3(C / 16), 0, e->st, Out); \
+		hipLaunchKernelGGL((k_colC<L_, R_>), gcol, dim3(G::threads), 0, e->st, Out, 1); \
+		h = hipGetLastError(); break; }
+	CT_MUL_CASE(10, 4) CT_MUL_CASE(11, 4) CT_MUL_CASE(12, 4)
+	CT_MUL_CASE(13, 8) CT_MUL_CASE(14, 8) CT_MUL_CASE(15, 8) CT_MUL_CASE(16, 8) CT_MUL_CASE(17, 8)
+	CT_MUL_CASE(18, 8) CT_MUL_CASE(19, 8) CT_MUL_CASE(20, 8) CT_MUL_CASE(21, 8) CT_MUL_CASE(22, 8)
+#undef CT_MUL_CASE
+	default: h = hipErrorInvalidValue;
+	}
+	HIPCHK(h);
+	// materialise dst at once (only register 0 may carry pending carries)
+	const size_t total = (size_t)e->lanes * (e->n / 16);
+	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,
+	                   e->reg(dst), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes);
+	HIPCHK(hipGetLastError());
+	return CT_OK;
+}
+
+}   // namespace ct

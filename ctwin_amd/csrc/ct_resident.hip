@@ -1,0 +1,207 @@
+// Resident squaring kernels for n = 2^LN <= 2^14 digits: one workgroup owns one
+// candidate for many consecutive squarings and keeps the whole state on chip.
+//
+// Each of the T = n/16 threads holds 16 elements per prime in VGPRs.  One
+// squaring (the work of engine::squareDup, cyclo/src/engine.h:387-419) is, per
+// prime: radix-16 register passes over the levels 1..4a of the DIF tree
+// (a = floor((LN-1)/4)) with LDS transposes between them, a final register
+// pass over the last (LN-1) mod 4 levels fused with the pointwise square (the
+// sqr2 form of kernel.cl:189-194), and the mirrored inverse passes ending in
+// the level-1 merge.  Then Garner CRT and the signed base-b carry in 16-digit
+// chunks (normalize1/normalize2, kernel.cl:1029-1081) run on consecutive
+// digits after one more LDS transpose, and the carried digits return to the
+// transform layout.  Nothing but the exponent bits is read from HBM inside
+// the step loop.
+//
+// Layouts (t = thread, q = slot 0..15):
+//   pass L < a : k = j*m + i + q*m/16, m = n/16^L, i = t mod m/16, j = t div m/16
+//   L == a     : k = 16 t + q  (final pass, and the carry chunks)
+// Layout 0 (k = t + q*T) is also the layout of the digits in registers and in
+// HBM between launches, so loads and stores are coalesced.
+#include <hip/hip_runtime.h>
+
+#include "ct_arith.h"
+#include "ct_tables.h"
+#include "ct_ntt.h"
+#include "ct_kernels.h"
+
+namespace ct {
+
+// x <- x^2 (or x*y) mod (X^n - X^(n/2) + 1, p_P); the signed digits are read
+// from LDS (layout 0, region D), the output residues (exact coefficients mod
+// p) are left in registers in layout 0.
+template <int LN, int P, bool MUL>
+__device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[E], uint32_t (&out)[E],
+                                          uint32_t *lds, const tw_entry *tw, uint32_t t)
+{
+	constexpr uint32_t p = kP[P], n = geo<LN>::n;
+	const tw_entry *twf = tw + (size_t)(P * 2 + 0) * n, *twi = tw + (size_t)(P * 2 + 1) * n;
+	uint32_t u[E], v[E];
+	const int32_t *dr = D + lbase<LN, 0>(t);
+#pragma unroll
+	for (int k = 0; k < E; ++k) u[k] = from_signed(dr[loff<LN, 0>(k)], p);
+	fwd_chain<LN, P, 0, true>(u, lds, twf, 1u, t);
+	if (MUL)
+	{
+#pragma unroll
+		for (int k = 0; k < E; ++k) v[k] = from_signed(yd[k], p);
+		fwd_chain<LN, P, 0, true>(v, lds, twf, 1u, t);
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < E; ++k) v[k] = 0;
+	}
+	final_pass<LN, P, MUL>(u, v, twf, twi, 1u, t);
+	inv_chain<LN, P, geo<LN>::a - 1, true>(u, lds, twi, 1u, t);
+#pragma unroll
+	for (int k = 0; k < E; ++k) out[k] = u[k];
+}
+
+// One squaring (MUL = false, x <- x^2 * 2^dup) or product (MUL = true,
+// x <- x * y) with CRT and the chunked carry.  x lives in LDS region D
+// (layout 0) before and after; y (MUL only) in registers, layout 0.
+template <int LN, bool MUL>
+__device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool dup, const divisor &dv,
+                                     uint32_t *lds, int64_t *fbuf, const tw_entry *tw, const garner_consts &gc, uint32_t t)
+{
+	constexpr int a = geo<LN>::a;
+	constexpr uint32_t T = geo<LN>::T;
+	uint32_t r0[E], r1[E], r2[E];
+	transform<LN, 0, MUL>(D, yd, r0, lds, tw, t);
+	transform<LN, 1, MUL>(D, yd, r1, lds, tw, t);
+	transform<LN, 2, MUL>(D, yd, r2, lds, tw, t);
+	xchg<LN, 0, a>(r0, lds, t);
+	xchg<LN, 0, a>(r1, lds, t);
+	xchg<LN, 0, a>(r2, lds, t);
+
+	// normalize1: CRT, optional doubling, signed digits of 16 consecutive coefficients
+	int32_t dig[E];
+	i96 l[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q) l[q] = garner(r0[q], r1[q], r2[q], gc);
+	fbuf[t] = carry_chunk(l, dig, dup, dv);
+	__syncthreads();
+	// normalize2: carry in from the previous chunk; X^n = X^(n/2) - 1 wraps the
+	// top carry to chunk 0 (negated) and to chunk T/2 (added)
+	int64_t c = fbuf[(t + T - 1) % T];
+	if (t == 0) c = -c;
+	else if (t == T / 2) c += fbuf[T - 1];
+	carry_in(dig, c, dv);
+	// the carried chunk goes straight back to D (every reader of D in this
+	// step has passed the barriers above)
+	int32_t *dw = D + lbase<LN, a>(t);
+#pragma unroll
+	for (int q = 0; q < E; ++q) dw[loff<LN, a>(q)] = dig[q];
+	__syncthreads();
+}
+
+// ---- kernels -----------------------------------------------------------------
+
+template <int LN>
+__global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
+{
+	constexpr uint32_t n = geo<LN>::n, T = geo<LN>::T;
+	__shared__ uint32_t lds[geo<LN>::lds_words];
+	__shared__ int32_t D[geo<LN>::lds_words];
+	__shared__ int64_t fbuf[T];
+	const uint32_t c = blockIdx.x, t = threadIdx.x;
+	int32_t *x = A.x + (size_t)c * n;
+	{
+		int32_t *dw = D + lbase<LN, 0>(t);
+#pragma unroll
+		for (int q = 0; q < E; ++q) dw[loff<LN, 0>(q)] = x[t + q * T];
+	}
+	__syncthreads();
+	const divisor dv = A.dv[c];
+	const uint32_t *bits = A.bits ? A.bits + (size_t)c * A.bit_words : nullptr;
+	const bool mdup = A.mask ? ((A.mask[c >> 5] >> (c & 31)) & 1u) : false;
+	int32_t none[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q) none[q] = 0;
+	for (int it = 0; it < A.nsteps; ++it)
+	{
+		const int i = A.i_hi - it;
+		const bool dup = bits ? ((bits[i >> 5] >> (i & 31)) & 1u) : mdup;
+		step<LN, false>(D, none, dup, dv, lds, fbuf, A.tw, A.gc, t);
+	}
+	const int32_t *dr = D + lbase<LN, 0>(t);
+#pragma unroll
+	for (int q = 0; q < E; ++q) x[t + q * T] = dr[loff<LN, 0>(q)];
+}
+
+template <int LN>
+__global__ __launch_bounds__(geo<LN>::T) void k_res_mul(mul_args A)
+{
+	constexpr uint32_t n = geo<LN>::n, T = geo<LN>::T;
+	__shared__ uint32_t lds[geo<LN>::lds_words];
+	__shared__ int32_t D[geo<LN>::lds_words];
+	__shared__ int64_t fbuf[T];
+	const uint32_t c = blockIdx.x, t = threadIdx.x;
+	const int32_t *xa = A.a + (size_t)c * n, *xb = A.b + (size_t)c * n;
+	int32_t yd[E];
+	{
+		int32_t *dw = D + lbase<LN, 0>(t);
+#pragma unroll
+		for (int q = 0; q < E; ++q) { dw[loff<LN, 0>(q)] = xa[t + q * T]; yd[q] = xb[t + q * T]; }
+	}
+	__syncthreads();
+	const divisor dv = A.dv[c];
+	step<LN, true>(D, yd, false, dv, lds, fbuf, A.tw, A.gc, t);
+	int32_t *xo = A.out + (size_t)c * n;
+	const int32_t *dr = D + lbase<LN, 0>(t);
+#pragma unroll
+	for (int q = 0; q < E; ++q) xo[t + q * T] = dr[loff<LN, 0>(q)];
+}
+
+template <int LN>
+static hipError_t launch_steps_ln(const res_args &A, uint32_t lanes, hipStream_t st)
+{
+	hipLaunchKernelGGL(k_res_steps<LN>, dim3(lanes), dim3(geo<LN>::T), 0, st, A);
+	return hipGetLastError();
+}
+
+template <int LN>
+static hipError_t launch_mul_ln(const mul_args &A, uint32_t lanes, hipStream_t st)
+{
+	hipLaunchKernelGGL(k_res_mul<LN>, dim3(lanes), dim3(geo<LN>::T), 0, st, A);
+	return hipGetLastError();
+}
+
+hipError_t resident_steps(int ln, const res_args &A, uint32_t lanes, hipStream_t st)
+{
+	switch (ln)
+	{
+	case 5: return launch_steps_ln<5>(A, lanes, st);
+	case 6: return launch_steps_ln<6>(A, lanes, st);
+	case 7: return launch_steps_ln<7>(A, lanes, st);
+	case 8: return launch_steps_ln<8>(A, lanes, st);
+	case 9: return launch_steps_ln<9>(A, lanes, st);
+	case 10: return launch_steps_ln<10>(A, lanes, st);
+	case 11: return launch_steps_ln<11>(A, lanes, st);
+	case 12: return launch_steps_ln<12>(A, lanes, st);
+	case 13: return launch_steps_ln<13>(A, lanes, st);
+	case 14: return launch_steps_ln<14>(A, lanes, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+hipError_t resident_mul(int ln, const mul_args &A, uint32_t lanes, hipStream_t st)
+{
+	switch (ln)
+	{
+	case 5: return launch_mul_ln<5>(A, lanes, st);
+	case 6: return launch_mul_ln<6>(A, lanes, st);
+	case 7: return launch_mul_ln<7>(A, lanes, st);
+	case 8: return launch_mul_ln<8>(A, lanes, st);
+	case 9: return launch_mul_ln<9>(A, lanes, st);
+	case 10: return launch_mul_ln<10>(A, lanes, st);
+	case 11: return launch_mul_ln<11>(A, lanes, st);
+	case 12: return launch_mul_ln<12>(A, lanes, st);
+	case 13: return launch_mul_ln<13>(A, lanes, st);
+	case 14: return launch_mul_ln<14>(A, lanes, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+}   // namespace ct

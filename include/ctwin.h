@@ -1,0 +1,130 @@
+/*
+ * ctwin_amd -- C ABI of the MI355X (gfx950) engines for galloty/ctwin's two
+ * data-parallel paths: the exact Fermat squaring engine for
+ * N = b^n - b^(n/2) + 1 (n = 2^m digits in base b) and the b-window sieve.
+ *
+ * Plain C types only; every call returns a ct_status (0 = success) and never
+ * exits the process (the reference's engine calls exit() on errors,
+ * cyclo/src/ocl.h:106-113).  One engine per host thread; calls on one engine
+ * are stream-ordered and only the read-out calls block.
+ */
+#ifndef CTWIN_H
+#define CTWIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum
+{
+	CT_OK = 0,
+	CT_ERR_ARG = -1,          /* invalid argument */
+	CT_ERR_HIP = -2,          /* HIP runtime error (no device, launch failure) */
+	CT_ERR_NOMEM = -3,        /* device allocation failed */
+	CT_ERR_UNSUPPORTED = -4,  /* size outside the engine's range */
+	CT_ERR_GETINT = -5,       /* canonical read-out failed ("getInt failed", transform.h:589) */
+	CT_ERR_COMPUTE = -6,      /* residue 0 ("computation failed", transform.h:611 / cyclo.h:369) */
+	CT_ERR_GERBICZ = -7       /* Gerbicz-Li mismatch ("Gerbicz failed", cyclo.h:377) */
+} ct_status;
+
+typedef struct ct_engine ct_engine;
+
+/* Engine kinds. */
+#define CT_KIND_AUTO      0   /* resident kernels for n <= 2^14, else multipass */
+#define CT_KIND_RESIDENT  1   /* one workgroup per candidate, state on chip (n <= 2^14) */
+#define CT_KIND_PLAIN     2   /* per-squaring launch sequence over the whole GPU */
+
+/* Replaces engine(platform, d, verbose) + transform(size, engine, isBoinc,
+ * csize) (cyclo/src/engine.h:40, cyclo/src/transform.h:380-433): device
+ * ordinal, log2 n (5..23), number of candidates ("lanes"; the reference
+ * fixes VSIZE = 64, engine.h:19), carry chunk csize (0 or 16; the reference
+ * autotunes 8..64, transform.h:383-419; the carried digits depend on it, the
+ * residues do not), engine kind.  Allocates 3 digit registers per lane. */
+int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out);
+int ct_engine_destroy(ct_engine *e);
+
+/* transform::set_b (transform.h:472-486): b[lanes], 32 < b < 2^31. */
+int ct_engine_set_b(ct_engine *e, const uint32_t *b);
+
+/* engine::set (engine.h:231-237, kernel.cl:514-520): register 0 := a. */
+int ct_engine_set(ct_engine *e, uint32_t a);
+
+/* engine::copy (engine.h:240-247): register dst := register src (0..2). */
+int ct_engine_copy(ct_engine *e, uint32_t dst, uint32_t src);
+
+/* engine::squareDup (engine.h:387-419): register 0 := x^2 * 2^bit(lane),
+ * bit of lane l = (dup[l / 64] >> (l % 64)) & 1; dup may be NULL (no doubling). */
+int ct_engine_square_dup(ct_engine *e, const uint64_t *dup);
+
+/* engine::mul (engine.h:421-433): register 0 := register 0 * register 1.
+ * Unlike the reference, register 1 is preserved. */
+int ct_engine_mul(ct_engine *e);
+
+/* Same product into any register: dst := a * b. */
+int ct_engine_mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
+
+/* transform::getInt (transform.h:501-597): canonical digits in [0, b) of
+ * register reg (the reference always reads device register 0) as
+ * digits[lane * n + j]. */
+int ct_engine_get_int(ct_engine *e, uint32_t reg, uint32_t *digits);
+
+/* transform::isPrime (transform.h:600-636) over canonical digits:
+ * prm[lane] = (value == 1), res64[lane] = value mod 2^64; returns
+ * CT_ERR_COMPUTE when some value is 0.  Host-only. */
+int ct_engine_is_prime(const ct_engine *e, const uint32_t *digits, int *prm, uint64_t *res64);
+
+/* Raw signed-digit state of a register (readMemory_x123 / writeMemory_x123,
+ * engine.h:204-225): digits[lane * n + j], the same non-canonical digits as
+ * the reference's device registers for the same chunk size. */
+int ct_engine_read_state(ct_engine *e, uint32_t reg, int32_t *digits);
+int ct_engine_write_state(ct_engine *e, uint32_t reg, const int32_t *digits);
+
+/* Device-resident exponent bits: bits[lane * words + w], bit i of lane l is
+ * (bits[l * words + i / 32] >> (i % 32)) & 1.  Replaces the per-iteration
+ * host mpz_tstbit loop of cyclo.h:244-249. */
+int ct_engine_set_exponents(ct_engine *e, const uint32_t *bits, uint32_t words);
+
+/* The PRP loop of cyclo.h:277-308 on the device: for i = i_hi down to i_lo,
+ * register 0 := x^2 * 2^(bit i); when gl_block > 0, after the step for bit i
+ * with i % gl_block == 0 and i / gl_block != 0, register 1 := register 0 *
+ * register 1 (Gerbicz-Li d(t)). */
+int ct_engine_run(ct_engine *e, int i_hi, int i_lo, int gl_block);
+
+/* cyclo::check for one batch (cyclo.h:252-411): base-2 Fermat PRP of
+ * b^n - b^(n/2) + 1 for every lane with Gerbicz-Li verification.  Outputs
+ * per lane: prm (probable prime), res64 and gl_ok; the canonical residues
+ * (optional, may be NULL) as residue[lane * n + j].  Returns CT_ERR_GERBICZ if
+ * any lane fails verification, CT_ERR_COMPUTE if any residue is 0. */
+int ct_prp_check(ct_engine *e, const uint32_t *b, int *prm, uint64_t *res64, int *gl_ok, uint32_t *residue);
+
+/* Wall time of the device work issued since the last call (ms), measured with
+ * HIP events on the engine's stream. */
+int ct_engine_timer_start(ct_engine *e);
+int ct_engine_timer_stop(ct_engine *e, float *ms);
+
+/* Engine facts: n, lanes, kind actually used. */
+int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind);
+
+/* ---------------- window sieve ---------------- */
+
+/* Sieve::check_pos / check_neg (sieve/CPU/ctsieve.cpp:678-787) on the GPU:
+ * strike every b in [b_min, b_max] with a prime factor p of
+ * b^(2^m) - b^(2^(m-1)) + 1 (p = 3k 2^m + 1, p_min_pos <= p up to the first
+ * p >= p_max_pos; mode >= 0) or of b^(2^m) - b^(2^(m-1)) - 1 (p = 10k +/- 1,
+ * same bounds rule with p_min_neg / p_max_neg; mode <= 0).  struck[b - b_min]
+ * is read and updated (1 = struck); p < 2^32.  The survivor list is the set of
+ * unstruck b in ascending order (cand_*.txt of ctsieve.cpp:621-626). */
+int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_max_pos,
+                    uint64_t p_min_neg, uint64_t p_max_neg, int mode, uint8_t *struck);
+
+/* Last error text of this thread (static storage). */
+const char *ct_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -503,6 +503,18 @@ void or_prp_steps(void *ve, int i0, int steps, int32_t *digits)
 	free(E);
 }
 
+void or_prp_steps_window(void *ve, const uint32_t *win, int i_lo_win, int i0, int steps, int32_t *digits)
+{
+	or_engine *e = (or_engine *)ve;
+	memset(digits, 0, sizeof(int32_t) * e->n); digits[0] = 1;
+	for (int t = 0, i = i0; t < steps && i >= 0; ++t, --i)
+	{
+		const int o = i - i_lo_win;
+		const int bit = (o >= 0) ? (int)((win[o / 32] >> (o % 32)) & 1u) : 0;
+		or_engine_square_dup(e, bit, digits);
+	}
+}
+
 int or_prp(void *ve, int i0, int flip_at, uint32_t *canon, uint64_t *res64, int *prm, int *gl_ok)
 {
 	or_engine *e = (or_engine *)ve;

@@ -113,6 +113,11 @@ int or_prp(void *e, int i0, int flip_at, uint32_t *canon, uint64_t *res64, int *
  * The signed digit state is returned. */
 void or_prp_steps(void *e, int i0, int steps, int32_t *digits);
 
+/* Same loop with the exponent bits given as a window: bit i of E is
+ * (win[(i - i_lo_win) / 32] >> ((i - i_lo_win) % 32)) & 1 for i >= i_lo_win
+ * (for sizes where computing E in full would be too slow). */
+void or_prp_steps_window(void *e, const uint32_t *win, int i_lo_win, int i0, int steps, int32_t *digits);
+
 /* ---------------- path B: window sieve ---------------- */
 
 int or_jacobi(uint64_t x, uint64_t y);

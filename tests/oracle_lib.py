@@ -50,6 +50,7 @@ def lib():
         L.or_prp.argtypes = [c.c_void_p, c.c_int, c.c_int, u32p, c.POINTER(c.c_uint64), c.POINTER(c.c_int), c.POINTER(c.c_int)]
         L.or_prp.restype = c.c_int
         L.or_prp_steps.argtypes = [c.c_void_p, c.c_int, c.c_int, i32p]
+        L.or_prp_steps_window.argtypes = [c.c_void_p, u32p, c.c_int, c.c_int, c.c_int, i32p]
         L.or_jacobi.argtypes = [c.c_uint64, c.c_uint64]; L.or_jacobi.restype = c.c_int
         L.or_isprime64.argtypes = [c.c_uint64]; L.or_isprime64.restype = c.c_int
         L.or_sieve_window.argtypes = [c.c_int, c.c_uint64, c.c_uint64, c.c_uint64, c.c_uint64,
@@ -106,6 +107,40 @@ class Engine:
         d = np.zeros(self.n, np.int32)
         lib().or_prp_steps(self.h, i0, steps, d)
         return d
+
+
+    def prp_steps_window(self, top_value, i_lo, i0, steps):
+        """top_value = E >> i_lo (the window of E's bits from i_lo up)."""
+        nbytes = (top_value.bit_length() + 7) // 8 + 4
+        win = np.frombuffer(top_value.to_bytes(nbytes + (-nbytes) % 4, "little"), np.uint32).copy()
+        d = np.zeros(self.n, np.int32)
+        lib().or_prp_steps_window(self.h, win, i_lo, i0, steps, d)
+        return d
+
+
+def top_bits_of_exponent(b, n, k):
+    """(E >> s, s) with E = b^n - b^(n/2), n a power of two, E >> s holding the
+    top k bits exactly (certified truncated squaring; raises when unsure)."""
+    P = k + 96
+    m = n.bit_length() - 1
+    lo, hi, e = b, b, 0          # b^(2^i) in [lo, hi] * 2^e
+    for _ in range(m):
+        lo, hi, e = lo * lo, hi * hi, 2 * e
+        sh = max(0, hi.bit_length() - P)
+        if sh:
+            lo, hi, e = lo >> sh, (hi >> sh) + 1, e + sh
+    # E = b^n - b^(n/2): b^(n/2) < 2^(bits(b^n)/2 + 1) is far below the window
+    top_len = hi.bit_length()
+    assert lo.bit_length() == top_len, "uncertain length"
+    s_local = top_len - k
+    a, c = lo >> s_local, hi >> s_local
+    assert a == c, "top bits uncertain"
+    half_bits = (n // 2) * b.bit_length() + 1
+    assert e + s_local > half_bits + 64
+    # the subtraction cannot borrow into the window unless the bits between
+    # the window and b^(n/2) are all zero: they are not (checked on lo)
+    assert (lo & ((1 << s_local) - 1)) != 0
+    return a, e + s_local
 
 
 def get_int(n, b, digits):
