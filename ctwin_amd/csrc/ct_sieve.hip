@@ -1,10 +1,511 @@
-// GPU window sieve (placeholder until the kernels land).
+// Window sieve for b^(2^m) - b^(2^(m-1)) +/- 1 on the GPU (the work of
+// Sieve::check_pos / check_neg, sieve/CPU/ctsieve.cpp:678-787).
+//
+// A b in [b_min, b_max] is struck when some prime p of the side's form in the
+// processed range divides N+(b) (p = 3k 2^m + 1) or N-(b) (p = 10k +/- 1):
+//   + : b mod p is a primitive 3*2^m-th root of unity,
+//   - : y = b^(2^(m-1)) mod p is a root of y^2 - y - 1, i.e. y = (1 +/- sqrt 5)/2.
+// The reference filters p with a base-2 Euler test and verifies every strike;
+// a composite p that survives it can only strike b that share a smaller prime
+// factor with it, so the survivor set is the same as with exact primes.  Here
+// primality is exact (Miller-Rabin to bases 2, 7, 61 is deterministic below
+// 4,759,123,141) and the roots are computed directly:
+//   + : c = a^k, k = (p-1)/(3 2^m), for the first a with c^(3 2^(m-1)) = -1
+//       and c^(2^(m-1)) != -1 (then c has order 3 2^m); roots c g^t and c^5 g^t,
+//       g = c^6, t < 2^(m-1).
+//   - : x^(2^d) = y (d = m-1) solved once (y^alpha with alpha 2^d = 1 mod the
+//       odd part of p-1, then a 2-group discrete logarithm), all 2^e roots by
+//       a primitive 2^e-th root of unity (e = min(v2(p-1), d)).
+// Pipeline per block of k: candidate kernel (trial division + Miller-Rabin,
+// wave-aggregated compaction of the primes), root kernels that write 1 into
+// struck[b - b_min] (idempotent plain byte stores; no atomics needed).  Primes
+// below 2^16 on the - side strike too many b per root for one thread: their
+// (p, root) pairs are queued and struck by one workgroup each.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
 
 #include "../../include/ctwin.h"
 #include "ct_engine_impl.h"
 
-int ct_sieve_window(int, int, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, int, uint8_t *)
+namespace ct {
+
+// ---------------------------------------------------------------------------
+// modular arithmetic: 32-bit Montgomery on the device (odd p < 2^32),
+// plain 128-bit arithmetic on the host (odd p < 2^63); same interface.
+
+struct m32
 {
-	return ct::fail(CT_ERR_UNSUPPORTED, "sieve not built yet");
+	uint32_t p, q, one, r2;
+
+	__device__ explicit m32(uint32_t p_) : p(p_)
+	{
+		uint32_t x = p;
+		for (int i = 0; i < 5; ++i) x *= 2u - p * x;
+		q = x;
+		one = (uint32_t)((0x100000000ull) % p);
+		r2 = (uint32_t)(((uint64_t)one * one) % p);
+	}
+	__device__ uint32_t mul(uint32_t a, uint32_t b) const
+	{
+		const uint64_t t = (uint64_t)a * b;
+		const uint32_t m = (uint32_t)t * q, th = (uint32_t)(t >> 32), mp = __umulhi(m, p);
+		const uint32_t r = th - mp;
+		return (th < mp) ? r + p : r;
+	}
+	__device__ uint32_t add(uint32_t a, uint32_t b) const { const uint32_t s = a + b; return (s < a || s >= p) ? s - p : s; }
+	__device__ uint32_t sub(uint32_t a, uint32_t b) const { return (a >= b) ? a - b : a - b + p; }
+	__device__ uint32_t to_m(uint32_t x) const { return mul(x % p, r2); }
+	__device__ uint32_t from_m(uint32_t x) const { return mul(x, 1u); }
+	__device__ uint32_t neg_one() const { return p - one; }
+	__device__ uint32_t pow(uint32_t a, uint64_t e) const
+	{
+		uint32_t r = one;
+		while (e) { if (e & 1) r = mul(r, a); a = mul(a, a); e >>= 1; }
+		return r;
+	}
+};
+
+struct m64h
+{
+	uint64_t p, one;
+	explicit m64h(uint64_t p_) : p(p_), one(1) {}
+	uint64_t mul(uint64_t a, uint64_t b) const { return (uint64_t)(((unsigned __int128)a * b) % p); }
+	uint64_t add(uint64_t a, uint64_t b) const { const unsigned __int128 s = (unsigned __int128)a + b; return (uint64_t)(s >= p ? s - p : s); }
+	uint64_t sub(uint64_t a, uint64_t b) const { return (a >= b) ? a - b : a + (p - b); }
+	uint64_t to_m(uint64_t x) const { return x % p; }
+	uint64_t from_m(uint64_t x) const { return x; }
+	uint64_t neg_one() const { return p - 1; }
+	uint64_t pow(uint64_t a, uint64_t e) const
+	{
+		uint64_t r = 1;
+		while (e) { if (e & 1) r = mul(r, a); a = mul(a, a); e >>= 1; }
+		return r;
+	}
+};
+
+#define CT_HD2 __host__ __device__ __forceinline__
+
+// Miller-Rabin with the given bases.
+template <class M, typename U>
+CT_HD2 bool mr_prime(const M &A, U p, const uint32_t *bases, int nb)
+{
+	U d = p - 1; int s = 0;
+	while ((d & 1) == 0) { d >>= 1; ++s; }
+	for (int i = 0; i < nb; ++i)
+	{
+		const U a = (U)bases[i] % p;
+		if (a == 0) continue;
+		U x = A.pow(A.to_m(a), d);
+		if (x == A.one || x == A.neg_one()) continue;
+		bool ok = false;
+		for (int r = 1; r < s; ++r) { x = A.mul(x, x); if (x == A.neg_one()) { ok = true; break; } }
+		if (!ok) return false;
+	}
+	return true;
+}
+
+// a * b mod q for a, b < q (q < 2^63 on the host, < 2^32 on the device)
+CT_HD2 uint64_t mulmod_q(uint64_t a, uint64_t b, uint64_t q)
+{
+	if (q <= 0xffffffffull) return (a * b) % q;
+	return (uint64_t)(((unsigned __int128)a * b) % q);
+}
+
+// Generator of the 2-Sylow subgroup (order 2^v), from the first quadratic
+// non-residue c: w = c^q, p - 1 = 2^v q.
+template <class M, typename U>
+CT_HD2 U two_sylow_gen(const M &A, U p, U q)
+{
+	for (uint32_t c = 2;; ++c)
+	{
+		const U cm = A.to_m((U)c);
+		if (A.pow(cm, (p - 1) / 2) == A.neg_one()) return A.pow(cm, q);
+	}
+}
+
+// One solution of x^(2^d) = y (Montgomery form), or 0 when there is none.
+// p - 1 = 2^v q, q odd, w generator of the 2-Sylow subgroup.
+template <class M, typename U>
+CT_HD2 U root_2d(const M &A, U p, U q, int v, U w, U y, int d)
+{
+	const int e = (v < d) ? v : d;
+	if (A.pow(y, (p - 1) >> e) != A.one) return 0;
+	// alpha = 2^-d mod q
+	uint64_t alpha = 1 % (uint64_t)q;
+	const uint64_t h = ((uint64_t)q + 1) / 2;     // 2^-1 mod q
+	for (int i = 0; i < d; ++i) alpha = mulmod_q(alpha, h, (uint64_t)q);
+	const U x1 = A.pow(y, alpha);
+	if (d >= v) return x1;          // y lies in the odd part: x1^(2^d) = y
+	// eta = y * x1^(-2^d) lies in the 2-Sylow subgroup and is a 2^d-th power there
+	U x1d = x1;
+	for (int i = 0; i < d; ++i) x1d = A.mul(x1d, x1d);
+	const U eta = A.mul(y, A.pow(x1d, p - 2));
+	// eta = W^k, W = w^(2^d) of order 2^L, L = v - d: bits of k from the bottom
+	const int L = v - d;
+	U W = w;
+	for (int i = 0; i < d; ++i) W = A.mul(W, W);
+	U Wi = A.pow(W, p - 2);         // W^-1, squared along
+	U mu = eta;
+	uint64_t k = 0;
+	for (int j = 0; j < L; ++j)
+	{
+		U t = mu;
+		for (int i = 0; i < L - 1 - j; ++i) t = A.mul(t, t);
+		if (t != A.one) { k |= (uint64_t)1 << j; mu = A.mul(mu, Wi); }
+		Wi = A.mul(Wi, Wi);
+	}
+	return A.mul(x1, A.pow(w, k));
+}
+
+// ---------------------------------------------------------------------------
+// striking
+
+__device__ __forceinline__ void strike_range(uint8_t *struck, uint64_t x, uint64_t p, uint64_t b_min, uint64_t b_max)
+{
+	uint64_t s = x;
+	if (s < b_min) s += ((b_min - s + p - 1) / p) * p;
+	for (; s <= b_max; s += p) struck[s - b_min] = 1;
+}
+
+__device__ __forceinline__ void strike_one(uint8_t *struck, uint32_t x, uint32_t p, uint64_t b_min, uint64_t b_max)
+{
+	if ((uint64_t)p > b_max) { if (x >= b_min && x <= b_max) struck[x - b_min] = 1; }
+	else strike_range(struck, x, p, b_min, b_max);
+}
+
+__device__ __forceinline__ bool small_factor(uint32_t p)
+{
+	return (p % 3 == 0) || (p % 5 == 0) || (p % 7 == 0) || (p % 11 == 0) || (p % 13 == 0) || (p % 17 == 0)
+	    || (p % 19 == 0) || (p % 23 == 0) || (p % 29 == 0) || (p % 31 == 0) || (p % 37 == 0) || (p % 41 == 0);
+}
+
+__device__ __forceinline__ bool is_prime32(uint32_t p)
+{
+	if (p < 2) return false;
+	if (p < 43) return p == 2 || p == 3 || p == 5 || p == 7 || p == 11 || p == 13 || p == 17 || p == 19
+	                || p == 23 || p == 29 || p == 31 || p == 37 || p == 41;
+	if ((p & 1) == 0 || small_factor(p)) return false;
+	const uint32_t bases[3] = {2u, 7u, 61u};
+	m32 A(p);
+	return mr_prime<m32, uint32_t>(A, p, bases, 3);
+}
+
+// wave-aggregated append of p to list (one atomic per wave)
+__device__ __forceinline__ void append(bool keep, uint32_t p, uint32_t *list, uint32_t *count)
+{
+	const uint64_t mask = __ballot(keep);
+	if (mask == 0) return;
+	const int lane = __lane_id();
+	const int leader = __ffsll((long long)mask) - 1;
+	uint32_t base = 0;
+	if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+	base = __shfl(base, leader);
+	if (keep) list[base + __popcll(mask & ((1ull << lane) - 1))] = p;
+}
+
+// ---------------------------------------------------------------------------
+// + side
+
+__global__ void k_primes_pos(uint64_t k0, uint64_t kcount, int m, uint32_t *list, uint32_t *count)
+{
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	bool keep = false;
+	uint32_t p = 0;
+	if (i < kcount)
+	{
+		const uint64_t pp = 3 * ((k0 + i) << m) + 1;
+		p = (uint32_t)pp;
+		keep = (pp < 0x100000000ull) && is_prime32(p);
+	}
+	append(keep, p, list, count);
+}
+
+// one workgroup per prime: find c, then 256 threads walk g = c^6
+__global__ void k_roots_pos(const uint32_t *list, const uint32_t *count, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck)
+{
+	const uint32_t idx = blockIdx.x;
+	if (idx >= *count) return;
+	const uint32_t p = list[idx];
+	const m32 A(p);
+	const uint64_t k = (uint64_t)(p - 1) / (3ull << m);
+	__shared__ uint32_t sc;
+	if (threadIdx.x == 0)
+	{
+		uint32_t c = 0;
+		for (uint32_t a = 2;; ++a)
+		{
+			c = A.pow(A.to_m(a), k);
+			uint32_t c2 = c;
+			for (int i = 0; i < m - 1; ++i) c2 = A.mul(c2, c2);           // c^(2^(m-1))
+			const uint32_t c6 = A.mul(A.mul(c2, c2), c2);                  // c^(3 2^(m-1))
+			if (c6 == A.neg_one() && c2 != A.neg_one()) break;
+		}
+		sc = c;
+	}
+	__syncthreads();
+	const uint32_t c = sc;
+	const uint32_t c2 = A.mul(c, c), c4 = A.mul(c2, c2), c5 = A.mul(c4, c), g = A.mul(c4, c2);
+	const uint32_t half = 1u << (m - 1);                 // t < 2^(m-1)
+	const uint32_t nt = blockDim.x;
+	const uint32_t per = (half + nt - 1) / nt;
+	const uint32_t t0 = threadIdx.x * per;
+	if (t0 >= half) return;
+	const uint32_t t1 = min(half, t0 + per);
+	uint32_t cur = A.pow(g, t0);
+	for (uint32_t t = t0; t < t1; ++t)
+	{
+		strike_one(struck, A.from_m(A.mul(c, cur)), p, b_min, b_max);
+		strike_one(struck, A.from_m(A.mul(c5, cur)), p, b_min, b_max);
+		cur = A.mul(cur, g);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// - side
+
+__global__ void k_primes_neg(uint64_t k0, uint64_t kcount, uint32_t *list, uint32_t *count)
+{
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	for (int s = 0; s < 2; ++s)
+	{
+		bool keep = false;
+		uint32_t p = 0;
+		if (i < kcount)
+		{
+			const uint64_t pp = 10 * (k0 + i) + 2 * s - 1;
+			p = (uint32_t)pp;
+			keep = (pp >= 11) && (pp < 0x100000000ull) && is_prime32(p);
+		}
+		append(keep, p, list, count);
+	}
+}
+
+constexpr uint32_t kSmallP = 1u << 16;
+
+// one thread per prime: sqrt(5), y = (1 +/- sqrt 5)/2, all 2^(m-1)-th roots of y
+__global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, uint64_t b_min, uint64_t b_max,
+                            uint8_t *struck, uint2 *small, uint32_t *small_count, uint32_t small_cap)
+{
+	const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= *count) return;
+	const uint32_t p = list[idx];
+	const m32 A(p);
+	uint32_t q = p - 1; int v = 0;
+	while ((q & 1) == 0) { q >>= 1; ++v; }
+	const uint32_t w = two_sylow_gen<m32, uint32_t>(A, p, q);
+	const uint32_t s5 = root_2d<m32, uint32_t>(A, p, q, v, w, A.to_m(5u), 1);
+	if (s5 == 0) return;
+	const uint32_t inv2 = A.to_m((p + 1) / 2);
+	const int d = m - 1, e = (v < d) ? v : d;
+	uint32_t u = w;                                        // primitive 2^e-th root: w^(2^(v-e))
+	for (int i = 0; i < v - e; ++i) u = A.mul(u, u);
+	for (int side = 0; side < 2; ++side)
+	{
+		const uint32_t y = A.mul(side ? A.add(A.one, s5) : A.sub(A.one, s5), inv2);
+		const uint32_t x = root_2d<m32, uint32_t>(A, p, q, v, w, y, d);
+		if (x == 0) continue;
+		uint32_t r = x;
+		const uint32_t cnt = 1u << e;
+		for (uint32_t j = 0; j < cnt; ++j)
+		{
+			const uint32_t b = A.from_m(r);
+			if (p < kSmallP)
+			{
+				const uint32_t slot = atomicAdd(small_count, 1u);
+				if (slot < small_cap) small[slot] = make_uint2(p, b);
+			}
+			else strike_one(struck, b, p, b_min, b_max);
+			r = A.mul(r, u);
+		}
+	}
+}
+
+// one workgroup per (p, root) of a small prime: threads stride the window
+__global__ void k_strike_small(const uint2 *small, const uint32_t *small_count, uint64_t b_min, uint64_t b_max, uint8_t *struck)
+{
+	if (blockIdx.x >= *small_count) return;
+	const uint2 e = small[blockIdx.x];
+	const uint64_t p = e.x;
+	uint64_t s = e.y;
+	if (s < b_min) s += ((b_min - s + p - 1) / p) * p;
+	for (uint64_t t = s + threadIdx.x * p; t <= b_max; t += (uint64_t)blockDim.x * p) struck[t - b_min] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// host fallback for the first p >= 2^32 that the reference's bound rule
+// includes (k_max is the first k with p(k) >= p_max, ctsieve.cpp:684, 734)
+
+static bool host_isprime(uint64_t p)
+{
+	if (p < 2) return false;
+	const uint32_t small[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+	for (uint32_t s : small) { if (p == s) return true; if (p % s == 0) return false; }
+	m64h A(p);
+	return mr_prime<m64h, uint64_t>(A, p, small, 12);   // deterministic for p < 3.3e24
+}
+
+static void host_strike(uint8_t *struck, uint64_t x, uint64_t p, uint64_t b_min, uint64_t b_max)
+{
+	uint64_t s = x;
+	if (s < b_min) s += ((b_min - s + p - 1) / p) * p;
+	for (; s <= b_max; s += p) struck[s - b_min] = 1;
+}
+
+static void host_pos(uint64_t p, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck)
+{
+	const m64h A(p);
+	const uint64_t k = (p - 1) / (3ull << m);
+	uint64_t c = 0;
+	for (uint64_t a = 2;; ++a)
+	{
+		c = A.pow(a, k);
+		uint64_t c2 = c;
+		for (int i = 0; i < m - 1; ++i) c2 = A.mul(c2, c2);
+		if (A.mul(A.mul(c2, c2), c2) == p - 1 && c2 != p - 1) break;
+	}
+	const uint64_t c2 = A.mul(c, c), c4 = A.mul(c2, c2), c5 = A.mul(c4, c), g = A.mul(c4, c2);
+	uint64_t cur = 1;
+	for (uint64_t t = 0; t < (1ull << (m - 1)); ++t)
+	{
+		host_strike(struck, A.mul(c, cur), p, b_min, b_max);
+		host_strike(struck, A.mul(c5, cur), p, b_min, b_max);
+		cur = A.mul(cur, g);
+	}
+}
+
+static void host_neg(uint64_t p, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck)
+{
+	const m64h A(p);
+	uint64_t q = p - 1; int v = 0;
+	while ((q & 1) == 0) { q >>= 1; ++v; }
+	const uint64_t w = two_sylow_gen<m64h, uint64_t>(A, p, q);
+	const uint64_t s5 = root_2d<m64h, uint64_t>(A, p, q, v, w, 5, 1);
+	if (s5 == 0) return;
+	const uint64_t inv2 = (p + 1) / 2;
+	const int d = m - 1, e = (v < d) ? v : d;
+	uint64_t u = w;
+	for (int i = 0; i < v - e; ++i) u = A.mul(u, u);
+	for (int side = 0; side < 2; ++side)
+	{
+		const uint64_t y = A.mul(side ? A.add(1, s5) : A.sub(1, s5), inv2);
+		const uint64_t x = root_2d<m64h, uint64_t>(A, p, q, v, w, y, d);
+		if (x == 0) continue;
+		uint64_t r = x;
+		for (uint64_t j = 0; j < (1ull << e); ++j) { host_strike(struck, r, p, b_min, b_max); r = A.mul(r, u); }
+	}
+}
+
+}   // namespace ct
+
+using namespace ct;
+
+#define HIPCHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) { rc = hip_fail(_e, #call); goto done; } } while (0)
+
+int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_max_pos,
+                    uint64_t p_min_neg, uint64_t p_max_neg, int mode, uint8_t *struck)
+{
+	if (!struck) return fail(CT_ERR_ARG, "struck is null");
+	if (m < 8 || m > 24) return fail(CT_ERR_ARG, "m must be in [8, 24] (ctsieve.cpp:841)");
+	if (b_max < b_min || b_max - b_min >= (1ull << 32)) return fail(CT_ERR_ARG, "bad b window");
+	const uint64_t P32 = 1ull << 32;
+	// k ranges with the reference's bound rule (ctsieve.cpp:680-684, 730-734)
+	const uint64_t kp_min = (p_min_pos >> m) / 3;
+	uint64_t kp_max = (p_max_pos >> m) / 3;
+	while (3 * (kp_max << m) + 1 < p_max_pos) ++kp_max;
+	const uint64_t kn_min = p_min_neg / 10;
+	uint64_t kn_max = p_max_neg / 10;
+	while (10 * kn_max + 1 < p_max_neg) ++kn_max;
+	if (mode >= 0 && 3 * (kp_max << m) + 1 >= P32 && 3 * ((kp_max - 1) << m) + 1 >= P32)
+		return fail(CT_ERR_UNSUPPORTED, "+ side: p beyond 2^32");
+	if (mode <= 0 && 10 * kn_max - 1 >= P32 && 10 * (kn_max - 1) + 1 >= P32)
+		return fail(CT_ERR_UNSUPPORTED, "- side: p beyond 2^32");
+
+	const size_t W = (size_t)(b_max - b_min + 1);
+	const uint64_t KB = 1ull << 24;                      // k values per block
+	int rc = CT_OK;
+	uint8_t *d_struck = nullptr;
+	uint32_t *d_list = nullptr, *d_count = nullptr, *d_scount = nullptr;
+	uint2 *d_small = nullptr;
+	const uint32_t small_cap = 1u << 22;
+	hipStream_t st = nullptr;
+	{
+		HIPCHK(hipSetDevice(device));
+		HIPCHK(hipStreamCreate(&st));
+		HIPCHK(hipMalloc((void **)&d_struck, W));
+		HIPCHK(hipMalloc((void **)&d_list, 2 * KB * sizeof(uint32_t)));
+		HIPCHK(hipMalloc((void **)&d_count, sizeof(uint32_t)));
+		HIPCHK(hipMalloc((void **)&d_scount, sizeof(uint32_t)));
+		HIPCHK(hipMalloc((void **)&d_small, small_cap * sizeof(uint2)));
+		HIPCHK(hipMemcpy(d_struck, struck, W, hipMemcpyHostToDevice));
+		if (mode >= 0 && kp_max >= kp_min)
+		{
+			for (uint64_t k0 = kp_min; k0 <= kp_max; k0 += KB)
+			{
+				const uint64_t kc = std::min(KB, kp_max - k0 + 1);
+				if (3 * (k0 << m) + 1 >= P32) break;
+				HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
+				hipLaunchKernelGGL(k_primes_pos, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, m, d_list, d_count);
+				HIPCHK(hipGetLastError());
+				uint32_t cnt = 0;
+				HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+				HIPCHK(hipStreamSynchronize(st));
+				if (cnt)
+				{
+					hipLaunchKernelGGL(k_roots_pos, dim3(cnt), dim3(256), 0, st, d_list, d_count, m, b_min, b_max, d_struck);
+					HIPCHK(hipGetLastError());
+				}
+			}
+		}
+		if (mode <= 0 && kn_max >= kn_min)
+		{
+			for (uint64_t k0 = kn_min; k0 <= kn_max; k0 += KB)
+			{
+				const uint64_t kc = std::min(KB, kn_max - k0 + 1);
+				if (10 * k0 >= P32 + 1) break;
+				HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
+				HIPCHK(hipMemsetAsync(d_scount, 0, sizeof(uint32_t), st));
+				hipLaunchKernelGGL(k_primes_neg, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, d_list, d_count);
+				HIPCHK(hipGetLastError());
+				uint32_t cnt = 0, scnt = 0;
+				HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+				HIPCHK(hipStreamSynchronize(st));
+				if (cnt)
+				{
+					hipLaunchKernelGGL(k_roots_neg, dim3((cnt + 127) / 128), dim3(128), 0, st, d_list, d_count, m, b_min, b_max,
+					                   d_struck, d_small, d_scount, small_cap);
+					HIPCHK(hipGetLastError());
+					HIPCHK(hipMemcpyAsync(&scnt, d_scount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+					HIPCHK(hipStreamSynchronize(st));
+					if (scnt > small_cap) { rc = fail(CT_ERR_UNSUPPORTED, "too many small-prime roots"); goto done; }
+					if (scnt)
+					{
+						hipLaunchKernelGGL(k_strike_small, dim3(scnt), dim3(256), 0, st, d_small, d_scount, b_min, b_max, d_struck);
+						HIPCHK(hipGetLastError());
+					}
+				}
+			}
+		}
+		HIPCHK(hipStreamSynchronize(st));
+		HIPCHK(hipMemcpy(struck, d_struck, W, hipMemcpyDeviceToHost));
+		// boundary primes >= 2^32 included by the reference's rule
+		if (mode >= 0 && kp_max >= kp_min)
+		{
+			const uint64_t p = 3 * (kp_max << m) + 1;
+			if (p >= P32 && host_isprime(p)) host_pos(p, m, b_min, b_max, struck);
+		}
+		if (mode <= 0 && kn_max >= kn_min)
+		{
+			for (int s = 0; s < 2; ++s)
+			{
+				const uint64_t p = 10 * kn_max + 2 * s - 1;
+				if (p >= P32 && host_isprime(p)) host_neg(p, m, b_min, b_max, struck);
+			}
+		}
+	}
+done:
+	hipFree(d_struck); hipFree(d_list); hipFree(d_count); hipFree(d_scount); hipFree(d_small);
+	if (st) hipStreamDestroy(st);
+	return rc;
 }

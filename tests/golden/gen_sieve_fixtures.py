@@ -1,0 +1,63 @@
+"""Writes the sieve survivor fixtures from the reference CPU sieve.
+
+The reference program is compiled from its own source by oracle/ref/Makefile
+into oracle/_ref/ctsieve_ref (this container only; it needs /root/reference).
+Each case records the window, the p bounds and the survivors: small cases
+keep the full ascending list (delta-encoded), the full window keeps the count,
+the SHA-256 of the cand_*.txt text ("b\\n" per survivor) and the first and
+last survivors.
+
+usage: python tests/golden/gen_sieve_fixtures.py [--full]
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref", "ctsieve_ref")
+
+SMALL = [
+    {"name": "m8_tiny", "m": 8, "b_min": 2, "b_max": 4000, "p_max_pos": 2000000, "p_max_neg": 100000},
+    {"name": "m10_small", "m": 10, "b_min": 1000000, "b_max": 1016383, "p_max_pos": 100000000, "p_max_neg": 10000000},
+    {"name": "m12_window64k", "m": 12, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 16) - 1,
+     "p_max_pos": 1000000000, "p_max_neg": 100000000},
+]
+FULL = {"name": "m12_config2", "m": 12, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 24) - 1,
+        "p_max_pos": 1 << 32, "p_max_neg": 1 << 32}
+
+
+def run(case):
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref")])
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "s.txt")
+        subprocess.check_call([REF, str(case["m"]), str(case["b_min"]), str(case["b_max"]),
+                               str(case["p_max_pos"]), str(case["p_max_neg"]), out], cwd=d,
+                              stdout=subprocess.DEVNULL)
+        text = open(out).read()
+    surv = [int(x) for x in text.split()]
+    return surv, hashlib.sha256(text.encode()).hexdigest()
+
+
+def main():
+    cases = []
+    for c in SMALL:
+        surv, h = run(c)
+        deltas = [surv[0]] + [b - a for a, b in zip(surv, surv[1:])] if surv else []
+        cases.append(dict(c, count=len(surv), sha256=h, deltas=deltas))
+        print(c["name"], len(surv), flush=True)
+    json.dump({"source": "sieve/CPU/ctsieve.cpp check_pos + check_neg from the ini start",
+               "cases": cases}, open(os.path.join(HERE, "sieve_small.json"), "w"))
+    if "--full" in sys.argv:
+        surv, h = run(FULL)
+        json.dump(dict(FULL, source="sieve/CPU/ctsieve.cpp check_pos + check_neg from the ini start",
+                       count=len(surv), sha256=h, head=surv[:32], tail=surv[-32:]),
+                  open(os.path.join(HERE, "sieve_config2.json"), "w"))
+        print(FULL["name"], len(surv))
+
+
+if __name__ == "__main__":
+    main()
