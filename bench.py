@@ -96,12 +96,28 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--big-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for hardware counters: 50 batched steps in one launch, 20 plain "
+                         "squarings at n=2^14 and n=2^21, and 3 copies of an 8 MiB register "
+                         "(calibration of FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the candidates shard one queue per GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import ctwin_amd as ct
+
+    if args.profile:
+        run_batched(ct, 50, 1)
+        b0 = random.Random(SEED).randrange(1 << 18, 1 << 19)
+        run_plain(ct, 14, b0, 20, 1)
+        run_plain(ct, 21, 65536 + 2 * random.Random(SEED).randrange(1 << 14), 20, 1)
+        t = ct.Transform(21, 1, ct.KIND_PLAIN)
+        for _ in range(3):
+            t.copy(1, 0)
+        t.timer_start(); t.timer_stop()
+        print("[bench] profile run done (copy kernel moves 8388608 bytes each way)", flush=True)
+        return
 
     sq_s, ms, i0, bs = run_batched(ct, args.steps, args.warmup)
     print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps)", flush=True)
