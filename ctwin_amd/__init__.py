@@ -28,7 +28,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_square_dup", "ct_engine_mul", "ct_engine_mul_regs", "ct_engine_get_int",
            "ct_engine_is_prime", "ct_engine_read_state", "ct_engine_write_state", "ct_engine_set_exponents",
            "ct_engine_run", "ct_prp_check", "ct_engine_timer_start", "ct_engine_timer_stop", "ct_engine_info",
-           "ct_sieve_window", "ct_last_error"]
+           "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes"]
 
 
 class CtError(RuntimeError):
@@ -74,6 +74,8 @@ def lib():
             "ct_engine_timer_stop": [vp, c.POINTER(c.c_float)],
             "ct_engine_info": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(i32)],
             "ct_sieve_window": [i32, i32, u64, u64, u64, u64, u64, u64, i32, u8p],
+            "ct_engine_set_primes": [vp, i32],
+            "ct_engine_primes": [vp, c.POINTER(i32)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -178,6 +180,16 @@ class Transform:
 
     def run(self, i_hi, i_lo=0, gl_block=0):
         _check(lib().ct_engine_run(self.h, int(i_hi), int(i_lo), int(gl_block)))
+
+    def set_primes(self, np_):
+        """0: automatic (2 primes when the coefficient bound allows), 3: always 3."""
+        _check(lib().ct_engine_set_primes(self.h, int(np_)))
+
+    @property
+    def primes(self):
+        v = ctypes.c_int()
+        _check(lib().ct_engine_primes(self.h, ctypes.byref(v)))
+        return v.value
 
     def timer_start(self):
         _check(lib().ct_engine_timer_start(self.h))

@@ -188,6 +188,7 @@ struct garner_consts
 	uint32_t p3_mod_p1, p3_mod_p2;
 	uint32_t m_lo, m_mid, m_hi;        // M = p1 p2 p3 (96-bit)
 	uint32_t h_lo, h_mid, h_hi;        // floor(M / 2)
+	uint64_t m2, h2;                   // two-prime mode: M2 = p1 p2, floor(M2 / 2)
 };
 
 }   // namespace ct

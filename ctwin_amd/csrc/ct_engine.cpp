@@ -125,6 +125,9 @@ int ct_engine_set_b(ct_engine *e, const uint32_t *b)
 	HIPCHK(hipStreamSynchronize(e->st));
 	HIPCHK(hipMemcpy(e->d_dv, dv.data(), dv.size() * sizeof(divisor), hipMemcpyHostToDevice));
 	e->b_set = true;
+	e->b_max = *std::max_element(e->b.begin(), e->b.end());
+	e->state_bound = 1;     // registers must be (re)initialised for the new modulus
+	choose_primes(e);
 	return CT_OK;
 }
 
@@ -134,6 +137,35 @@ int ct_engine_set(ct_engine *e, uint32_t a)
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(launch_set(e->d_x, e->n, e->lanes, (int32_t)a, e->st));
 	e->pending = false;
+	e->state_bound = std::max<uint64_t>(e->state_bound, a);
+	choose_primes(e);
+	return CT_OK;
+}
+
+// Every coefficient of a square (dup included) or product is bounded by
+// 3 n D^2, D >= |digit| (each product a_i a_k lands on at most two positions
+// of X^n - X^(n/2) + 1 and at most 1.5 n pairs meet at one position).  The
+// carry leaves |digit| <= b, so D = max(b, digits written or set).  Two primes
+// reconstruct exactly when 3 n D^2 < p1 p2 / 2.
+void ct::choose_primes(ct_engine *e)
+{
+	const uint64_t D = std::max<uint64_t>(e->b_max, e->state_bound);
+	const unsigned __int128 bound = (unsigned __int128)3 * e->n * D * D;
+	e->np = (e->force3 || bound >= e->gc.h2) ? 3 : 2;
+}
+
+int ct_engine_set_primes(ct_engine *e, int np)
+{
+	if (!e || (np != 0 && np != 3)) return fail(CT_ERR_ARG, "np must be 0 (automatic) or 3");
+	e->force3 = (np == 3);
+	choose_primes(e);
+	return CT_OK;
+}
+
+int ct_engine_primes(const ct_engine *e, int *np)
+{
+	if (!e || !np) return fail(CT_ERR_ARG, "bad argument");
+	*np = e->np;
 	return CT_OK;
 }
 
@@ -170,9 +202,20 @@ int ct_engine_read_state(ct_engine *e, uint32_t reg, int32_t *digits)
 int ct_engine_write_state(ct_engine *e, uint32_t reg, const int32_t *digits)
 {
 	if (!e || !digits || reg > 2) return fail(CT_ERR_ARG, "bad argument");
+	uint64_t dmax = 0;
+	for (uint32_t l = 0; l < e->lanes; ++l)
+		for (uint32_t j = 0; j < e->n; ++j)
+		{
+			const int64_t d = digits[(size_t)l * e->n + j];
+			const uint64_t ad = (uint64_t)(d < 0 ? -d : d);
+			if (e->b_set && ad > 2 * (uint64_t)e->b[l]) return fail(CT_ERR_ARG, "digits must satisfy |d| <= 2b");
+			dmax = std::max(dmax, ad);
+		}
 	HIPCHK(hipSetDevice(e->device));
 	int rc = settle(e);
 	if (rc) return rc;
+	e->state_bound = std::max(e->state_bound, dmax);
+	choose_primes(e);
 	HIPCHK(hipStreamSynchronize(e->st));
 	HIPCHK(hipMemcpy(e->reg(reg), digits, e->reg_bytes(), hipMemcpyHostToDevice));
 	return CT_OK;
@@ -212,7 +255,7 @@ int ct::steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, 
 	A.x = e->reg(0); A.tw = e->d_tw; A.dv = e->d_dv;
 	A.bits = bits; A.bit_words = bit_words; A.mask = mask_words;
 	A.i_hi = i_hi; A.nsteps = nsteps; A.gc = e->gc;
-	HIPCHK(resident_steps((int)e->ln, A, e->lanes, e->st));
+	HIPCHK(resident_steps((int)e->ln, e->np, A, e->lanes, e->st));
 	return CT_OK;
 }
 
@@ -221,7 +264,7 @@ int ct::mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
 	if (e->kind == CT_KIND_PLAIN) return plain_mul(e, dst, a, b);
 	mul_args A;
 	A.a = e->reg(a); A.b = e->reg(b); A.out = e->reg(dst); A.tw = e->d_tw; A.dv = e->d_dv; A.gc = e->gc;
-	HIPCHK(resident_mul((int)e->ln, A, e->lanes, e->st));
+	HIPCHK(resident_mul((int)e->ln, e->np, A, e->lanes, e->st));
 	return CT_OK;
 }
 

@@ -31,6 +31,10 @@ struct ct_engine
 	int64_t *d_carry = nullptr;       // [lanes][n/16] chunk carries
 	uint32_t rows_log2 = 0;           // column length R = 2^rows_log2
 	bool pending = false;             // register 0 still owes the chunk carries (plain engine)
+	int np = 3;                       // primes per squaring: 2 when 3 n D^2 < p1 p2 / 2
+	uint64_t b_max = 0;               // largest b over the lanes
+	uint64_t state_bound = 1;         // bound on |digit| set or written into the registers
+	bool force3 = false;              // always use three primes
 
 	int32_t *reg(uint32_t r) const { return d_x + (size_t)r * lanes * n; }
 	size_t reg_bytes() const { return (size_t)lanes * n * sizeof(int32_t); }
@@ -51,5 +55,6 @@ void plain_free(ct_engine *e);
 int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words);
 int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
 int plain_materialize(ct_engine *e);
+void choose_primes(ct_engine *e);
 
 }   // namespace ct

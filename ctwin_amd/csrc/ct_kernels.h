@@ -33,7 +33,7 @@ struct mul_args
 	garner_consts gc;
 };
 
-hipError_t resident_steps(int ln, const res_args &A, uint32_t lanes, hipStream_t st);
-hipError_t resident_mul(int ln, const mul_args &A, uint32_t lanes, hipStream_t st);
+hipError_t resident_steps(int ln, int np, const res_args &A, uint32_t lanes, hipStream_t st);
+hipError_t resident_mul(int ln, int np, const mul_args &A, uint32_t lanes, hipStream_t st);
 
 }   // namespace ct

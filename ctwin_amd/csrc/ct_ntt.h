@@ -288,6 +288,30 @@ __device__ __forceinline__ i96 garner(uint32_t r1, uint32_t r2, uint32_t r3, con
 	return r;
 }
 
+// Two-prime CRT (used when every coefficient is known to lie below
+// p1 p2 / 2 in magnitude): x = r2 + p2 ((r1 - r2) / p2 mod p1), centred.
+__device__ __forceinline__ int64_t garner2(uint32_t r1, uint32_t r2, const garner_consts &g)
+{
+	constexpr uint32_t p1 = kP[0], p2 = kP[1];
+	const uint32_t u = mont_mulc(sub_mod(r1, r2, p1), g.inv2_1, g.inv2_1q, p1);      // r2 < p2 < p1
+	const uint64_t v = (uint64_t)p2 * u + r2;                                          // < p1 p2 < 2^62
+	return (v > g.h2) ? (int64_t)(v - g.m2) : (int64_t)v;
+}
+
+// normalize1 with a 64-bit accumulator (two-prime mode: |l| < 2^61)
+__device__ __forceinline__ int64_t carry_chunk64(const int64_t (&l)[E], int32_t (&dig)[E], bool dup, const divisor &dv)
+{
+	int64_t acc = 0;
+#pragma unroll
+	for (int q = 0; q < E; ++q)
+	{
+		acc += l[q];
+		if (dup) acc += l[q];
+		dig[q] = divrem_i64(acc, dv);
+	}
+	return acc;
+}
+
 // normalize1 on one 16-coefficient chunk (kernel.cl:1029-1051): returns the
 // chunk's outgoing carry.
 __device__ __forceinline__ int64_t carry_chunk(const i96 (&l)[E], int32_t (&dig)[E], bool dup, const divisor &dv)

@@ -78,7 +78,7 @@ __device__ __forceinline__ void col_inv(uint32_t (&u)[E], uint32_t *X, const tw_
 	}
 }
 
-template <int LN, int RL>
+template <int LN, int RL, int NP>
 __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, uint32_t *res_lane, const tw_entry *tw,
                                                uint32_t c0, uint32_t cw, uint32_t tt, size_t plane)
 {
@@ -86,7 +86,7 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 	constexpr uint32_t n = 1u << LN, C = n >> RL;
 	constexpr int La = G::passes - 1;
 #pragma unroll
-	for (int P = 0; P < 3; ++P)
+	for (int P = 0; P < NP; ++P)
 	{
 		uint32_t u[E];
 		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
@@ -105,7 +105,7 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 }
 
 // A: chunk load (+ pending carry), column levels, residue planes
-template <int LN, int RL>
+template <int LN, int RL, int NP>
 __global__ __launch_bounds__(colgeo<RL>::threads) void k_colA(plain_args A, int operand)
 {
 	using G = colgeo<RL>;
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colA(plain_args A, int 
 	}
 	__syncthreads();
 	const uint32_t cw = t % G::W, tt = t / G::W;
-	colA_prime_all<LN, RL>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n);
+	colA_prime_all<LN, RL, NP>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n);
 }
 
 // B: row levels with the pointwise square / product, in place
@@ -185,7 +185,7 @@ __global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
 }
 
 // C: inverse column levels, CRT, normalize1
-template <int LN, int RL>
+template <int LN, int RL, int NP>
 __global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int no_dup)
 {
 	using G = colgeo<RL>;
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int 
 	const size_t plane = (size_t)A.lanes * n;
 	const uint32_t *res = A.res + (size_t)lane * n;
 #pragma unroll
-	for (int P = 0; P < 3; ++P)
+	for (int P = 0; P < NP; ++P)
 	{
 		uint32_t u[E];
 		const uint32_t *in = res + P * plane + c0 + cw;
@@ -219,13 +219,6 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int 
 	const uint32_t r = t / hpr, h = t % hpr;
 	const uint32_t k = r * C + c0 + 16 * h;
 	const uint32_t rr = r + (r >> 5);
-	i96 l[E];
-#pragma unroll
-	for (int q = 0; q < E; ++q)
-	{
-		const uint32_t a = (16 * h + q) * G::RP + rr;
-		l[q] = garner(S[0][a], S[1][a], S[2][a], A.gc);
-	}
 	bool dup = false;
 	if (!no_dup)
 	{
@@ -233,7 +226,29 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int 
 		else if (A.mask) dup = (A.mask[lane >> 5] >> (lane & 31)) & 1u;
 	}
 	int32_t dig[E];
-	const int64_t f = carry_chunk(l, dig, dup, A.dv[lane]);
+	int64_t f;
+	if (NP == 3)
+	{
+		i96 l[E];
+#pragma unroll
+		for (int q = 0; q < E; ++q)
+		{
+			const uint32_t a = (16 * h + q) * G::RP + rr;
+			l[q] = garner(S[0][a], S[1][a], S[2][a], A.gc);
+		}
+		f = carry_chunk(l, dig, dup, A.dv[lane]);
+	}
+	else
+	{
+		int64_t l[E];
+#pragma unroll
+		for (int q = 0; q < E; ++q)
+		{
+			const uint32_t a = (16 * h + q) * G::RP + rr;
+			l[q] = garner2(S[0][a], S[1][a], A.gc);
+		}
+		f = carry_chunk64(l, dig, dup, A.dv[lane]);
+	}
 	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * n + k);
 #pragma unroll
 	for (int v = 0; v < 4; ++v) dst[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
@@ -265,35 +280,64 @@ __global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, ui
 
 static int rows_log2(int ln) { return (ln <= 12) ? 4 : 8; }
 
-template <int LN, int RL>
+template <int LN, int RL, int NP>
 static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
 {
 	using G = colgeo<RL>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	hipLaunchKernelGGL((k_colA<LN, RL>), gcol, dim3(G::threads), 0, st, A, 0);
-	hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, 3, lanes), dim3(C / 16), 0, st, A);
-	hipLaunchKernelGGL((k_colC<LN, RL>), gcol, dim3(G::threads), 0, st, A, 0);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, A, 0);
+	hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
+	hipLaunchKernelGGL((k_colC<LN, RL, NP>), gcol, dim3(G::threads), 0, st, A, 0);
 	return hipGetLastError();
 }
 
-static hipError_t square_any(int ln, const plain_args &A, uint32_t lanes, hipStream_t st)
+template <int LN, int RL, int NP>
+static hipError_t mul_ln(const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+{
+	using G = colgeo<RL>;
+	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
+	const dim3 gcol(C / G::W, lanes);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, In, 0);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, In, 1);
+	hipLaunchKernelGGL((k_rowB<LN, RL, true>), dim3(R, NP, lanes), dim3(C / 16), 0, st, Out);
+	hipLaunchKernelGGL((k_colC<LN, RL, NP>), gcol, dim3(G::threads), 0, st, Out, 1);
+	return hipGetLastError();
+}
+
+template <int LN, int RL>
+static hipError_t square_np(int np, const plain_args &A, uint32_t lanes, hipStream_t st)
+{
+	return (np == 2) ? square_ln<LN, RL, 2>(A, lanes, st) : square_ln<LN, RL, 3>(A, lanes, st);
+}
+
+template <int LN, int RL>
+static hipError_t mul_np(int np, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+{
+	return (np == 2) ? mul_ln<LN, RL, 2>(In, Out, lanes, st) : mul_ln<LN, RL, 3>(In, Out, lanes, st);
+}
+
+#define CT_PLAIN_SIZES(X) X(10, 4) X(11, 4) X(12, 4) X(13, 8) X(14, 8) X(15, 8) X(16, 8) X(17, 8) \
+	X(18, 8) X(19, 8) X(20, 8) X(21, 8) X(22, 8)
+
+static hipError_t square_any(int ln, int np, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
 	switch (ln)
 	{
-	case 10: return square_ln<10, 4>(A, lanes, st);
-	case 11: return square_ln<11, 4>(A, lanes, st);
-	case 12: return square_ln<12, 4>(A, lanes, st);
-	case 13: return square_ln<13, 8>(A, lanes, st);
-	case 14: return square_ln<14, 8>(A, lanes, st);
-	case 15: return square_ln<15, 8>(A, lanes, st);
-	case 16: return square_ln<16, 8>(A, lanes, st);
-	case 17: return square_ln<17, 8>(A, lanes, st);
-	case 18: return square_ln<18, 8>(A, lanes, st);
-	case 19: return square_ln<19, 8>(A, lanes, st);
-	case 20: return square_ln<20, 8>(A, lanes, st);
-	case 21: return square_ln<21, 8>(A, lanes, st);
-	case 22: return square_ln<22, 8>(A, lanes, st);
+#define CT_SQ_CASE(L_, R_) case L_: return square_np<L_, R_>(np, A, lanes, st);
+	CT_PLAIN_SIZES(CT_SQ_CASE)
+#undef CT_SQ_CASE
+	default: return hipErrorInvalidValue;
+	}
+}
+
+static hipError_t mul_any(int ln, int np, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+{
+	switch (ln)
+	{
+#define CT_MUL_CASE(L_, R_) case L_: return mul_np<L_, R_>(np, In, Out, lanes, st);
+	CT_PLAIN_SIZES(CT_MUL_CASE)
+#undef CT_MUL_CASE
 	default: return hipErrorInvalidValue;
 	}
 }
@@ -347,7 +391,7 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 	{
 		A.i = i_hi - s;
 		A.carry_in = e->pending ? e->d_carry : nullptr;
-		HIPCHK(square_any((int)e->ln, A, e->lanes, e->st));
+		HIPCHK(square_any((int)e->ln, e->np, A, e->lanes, e->st));
 		e->pending = true;
 	}
 	return CT_OK;
@@ -364,24 +408,7 @@ int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
 	In.x = e->reg(a); In.xb = e->reg(b);
 	plain_args Out = make_args(e);
 	Out.x = e->reg(dst);
-	hipError_t h = hipSuccess;
-	switch ((int)e->ln)
-	{
-#define CT_MUL_CASE(L_, R_) case L_: { \
-		using G = colgeo<R_>; constexpr uint32_t C = 1u << (L_ - R_), R = 1u << R_; \
-		const dim3 gcol(C / G::W, e->lanes); \
-		hipLaunchKernelGGL((k_colA<L_, R_>), gcol, dim3(G::threads), 0, e->st, In, 0); \
-		hipLaunchKernelGGL((k_colA<L_, R_>), gcol, dim3(G::threads), 0, e->st, In, 1); \
-		hipLaunchKernelGGL((k_rowB<L_, R_, true>), dim3(R, 3, e->lanes), dim3(C / 16), 0, e->st, Out); \
-		hipLaunchKernelGGL((k_colC<L_, R_>), gcol, dim3(G::threads), 0, e->st, Out, 1); \
-		h = hipGetLastError(); break; }
-	CT_MUL_CASE(10, 4) CT_MUL_CASE(11, 4) CT_MUL_CASE(12, 4)
-	CT_MUL_CASE(13, 8) CT_MUL_CASE(14, 8) CT_MUL_CASE(15, 8) CT_MUL_CASE(16, 8) CT_MUL_CASE(17, 8)
-	CT_MUL_CASE(18, 8) CT_MUL_CASE(19, 8) CT_MUL_CASE(20, 8) CT_MUL_CASE(21, 8) CT_MUL_CASE(22, 8)
-#undef CT_MUL_CASE
-	default: h = hipErrorInvalidValue;
-	}
-	HIPCHK(h);
+	HIPCHK(mul_any((int)e->ln, e->np, In, Out, e->lanes, e->st));
 	// materialise dst at once (only register 0 may carry pending carries)
 	const size_t total = (size_t)e->lanes * (e->n / 16);
 	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,

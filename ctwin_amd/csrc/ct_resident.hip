@@ -59,9 +59,10 @@ __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[
 }
 
 // One squaring (MUL = false, x <- x^2 * 2^dup) or product (MUL = true,
-// x <- x * y) with CRT and the chunked carry.  x lives in LDS region D
-// (layout 0) before and after; y (MUL only) in registers, layout 0.
-template <int LN, bool MUL>
+// x <- x * y) with CRT and the chunked carry, over NP primes (NP = 2 when every
+// coefficient is known to be below p1 p2 / 2, else 3).  x lives in LDS region
+// D (layout 0) before and after; y (MUL only) in registers, layout 0.
+template <int LN, bool MUL, int NP>
 __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool dup, const divisor &dv,
                                      uint32_t *lds, int64_t *fbuf, const tw_entry *tw, const garner_consts &gc, uint32_t t)
 {
@@ -70,17 +71,27 @@ __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool du
 	uint32_t r0[E], r1[E], r2[E];
 	transform<LN, 0, MUL>(D, yd, r0, lds, tw, t);
 	transform<LN, 1, MUL>(D, yd, r1, lds, tw, t);
-	transform<LN, 2, MUL>(D, yd, r2, lds, tw, t);
+	if (NP == 3) transform<LN, 2, MUL>(D, yd, r2, lds, tw, t);
 	xchg<LN, 0, a>(r0, lds, t);
 	xchg<LN, 0, a>(r1, lds, t);
-	xchg<LN, 0, a>(r2, lds, t);
+	if (NP == 3) xchg<LN, 0, a>(r2, lds, t);
 
 	// normalize1: CRT, optional doubling, signed digits of 16 consecutive coefficients
 	int32_t dig[E];
-	i96 l[E];
+	if (NP == 3)
+	{
+		i96 l[E];
 #pragma unroll
-	for (int q = 0; q < E; ++q) l[q] = garner(r0[q], r1[q], r2[q], gc);
-	fbuf[t] = carry_chunk(l, dig, dup, dv);
+		for (int q = 0; q < E; ++q) l[q] = garner(r0[q], r1[q], r2[q], gc);
+		fbuf[t] = carry_chunk(l, dig, dup, dv);
+	}
+	else
+	{
+		int64_t l[E];
+#pragma unroll
+		for (int q = 0; q < E; ++q) l[q] = garner2(r0[q], r1[q], gc);
+		fbuf[t] = carry_chunk64(l, dig, dup, dv);
+	}
 	__syncthreads();
 	// normalize2: carry in from the previous chunk; X^n = X^(n/2) - 1 wraps the
 	// top carry to chunk 0 (negated) and to chunk T/2 (added)
@@ -98,7 +109,7 @@ __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool du
 
 // ---- kernels -----------------------------------------------------------------
 
-template <int LN>
+template <int LN, int NP>
 __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
 {
 	constexpr uint32_t n = geo<LN>::n, T = geo<LN>::T;
@@ -123,14 +134,14 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
 	{
 		const int i = A.i_hi - it;
 		const bool dup = bits ? ((bits[i >> 5] >> (i & 31)) & 1u) : mdup;
-		step<LN, false>(D, none, dup, dv, lds, fbuf, A.tw, A.gc, t);
+		step<LN, false, NP>(D, none, dup, dv, lds, fbuf, A.tw, A.gc, t);
 	}
 	const int32_t *dr = D + lbase<LN, 0>(t);
 #pragma unroll
 	for (int q = 0; q < E; ++q) x[t + q * T] = dr[loff<LN, 0>(q)];
 }
 
-template <int LN>
+template <int LN, int NP>
 __global__ __launch_bounds__(geo<LN>::T) void k_res_mul(mul_args A)
 {
 	constexpr uint32_t n = geo<LN>::n, T = geo<LN>::T;
@@ -147,7 +158,7 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_mul(mul_args A)
 	}
 	__syncthreads();
 	const divisor dv = A.dv[c];
-	step<LN, true>(D, yd, false, dv, lds, fbuf, A.tw, A.gc, t);
+	step<LN, true, NP>(D, yd, false, dv, lds, fbuf, A.tw, A.gc, t);
 	int32_t *xo = A.out + (size_t)c * n;
 	const int32_t *dr = D + lbase<LN, 0>(t);
 #pragma unroll
@@ -155,51 +166,53 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_mul(mul_args A)
 }
 
 template <int LN>
-static hipError_t launch_steps_ln(const res_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t launch_steps_ln(int np, const res_args &A, uint32_t lanes, hipStream_t st)
 {
-	hipLaunchKernelGGL(k_res_steps<LN>, dim3(lanes), dim3(geo<LN>::T), 0, st, A);
+	if (np == 2) hipLaunchKernelGGL((k_res_steps<LN, 2>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
+	else hipLaunchKernelGGL((k_res_steps<LN, 3>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
 	return hipGetLastError();
 }
 
 template <int LN>
-static hipError_t launch_mul_ln(const mul_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t launch_mul_ln(int np, const mul_args &A, uint32_t lanes, hipStream_t st)
 {
-	hipLaunchKernelGGL(k_res_mul<LN>, dim3(lanes), dim3(geo<LN>::T), 0, st, A);
+	if (np == 2) hipLaunchKernelGGL((k_res_mul<LN, 2>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
+	else hipLaunchKernelGGL((k_res_mul<LN, 3>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
 	return hipGetLastError();
 }
 
-hipError_t resident_steps(int ln, const res_args &A, uint32_t lanes, hipStream_t st)
+hipError_t resident_steps(int ln, int np, const res_args &A, uint32_t lanes, hipStream_t st)
 {
 	switch (ln)
 	{
-	case 5: return launch_steps_ln<5>(A, lanes, st);
-	case 6: return launch_steps_ln<6>(A, lanes, st);
-	case 7: return launch_steps_ln<7>(A, lanes, st);
-	case 8: return launch_steps_ln<8>(A, lanes, st);
-	case 9: return launch_steps_ln<9>(A, lanes, st);
-	case 10: return launch_steps_ln<10>(A, lanes, st);
-	case 11: return launch_steps_ln<11>(A, lanes, st);
-	case 12: return launch_steps_ln<12>(A, lanes, st);
-	case 13: return launch_steps_ln<13>(A, lanes, st);
-	case 14: return launch_steps_ln<14>(A, lanes, st);
+	case 5: return launch_steps_ln<5>(np, A, lanes, st);
+	case 6: return launch_steps_ln<6>(np, A, lanes, st);
+	case 7: return launch_steps_ln<7>(np, A, lanes, st);
+	case 8: return launch_steps_ln<8>(np, A, lanes, st);
+	case 9: return launch_steps_ln<9>(np, A, lanes, st);
+	case 10: return launch_steps_ln<10>(np, A, lanes, st);
+	case 11: return launch_steps_ln<11>(np, A, lanes, st);
+	case 12: return launch_steps_ln<12>(np, A, lanes, st);
+	case 13: return launch_steps_ln<13>(np, A, lanes, st);
+	case 14: return launch_steps_ln<14>(np, A, lanes, st);
 	default: return hipErrorInvalidValue;
 	}
 }
 
-hipError_t resident_mul(int ln, const mul_args &A, uint32_t lanes, hipStream_t st)
+hipError_t resident_mul(int ln, int np, const mul_args &A, uint32_t lanes, hipStream_t st)
 {
 	switch (ln)
 	{
-	case 5: return launch_mul_ln<5>(A, lanes, st);
-	case 6: return launch_mul_ln<6>(A, lanes, st);
-	case 7: return launch_mul_ln<7>(A, lanes, st);
-	case 8: return launch_mul_ln<8>(A, lanes, st);
-	case 9: return launch_mul_ln<9>(A, lanes, st);
-	case 10: return launch_mul_ln<10>(A, lanes, st);
-	case 11: return launch_mul_ln<11>(A, lanes, st);
-	case 12: return launch_mul_ln<12>(A, lanes, st);
-	case 13: return launch_mul_ln<13>(A, lanes, st);
-	case 14: return launch_mul_ln<14>(A, lanes, st);
+	case 5: return launch_mul_ln<5>(np, A, lanes, st);
+	case 6: return launch_mul_ln<6>(np, A, lanes, st);
+	case 7: return launch_mul_ln<7>(np, A, lanes, st);
+	case 8: return launch_mul_ln<8>(np, A, lanes, st);
+	case 9: return launch_mul_ln<9>(np, A, lanes, st);
+	case 10: return launch_mul_ln<10>(np, A, lanes, st);
+	case 11: return launch_mul_ln<11>(np, A, lanes, st);
+	case 12: return launch_mul_ln<12>(np, A, lanes, st);
+	case 13: return launch_mul_ln<13>(np, A, lanes, st);
+	case 14: return launch_mul_ln<14>(np, A, lanes, st);
 	default: return hipErrorInvalidValue;
 	}
 }

@@ -89,6 +89,7 @@ inline garner_consts make_garner()
 	const unsigned __int128 M = (unsigned __int128)p1 * p2 * p3, H = M / 2;
 	g.m_lo = (uint32_t)M; g.m_mid = (uint32_t)(M >> 32); g.m_hi = (uint32_t)(M >> 64);
 	g.h_lo = (uint32_t)H; g.h_mid = (uint32_t)(H >> 32); g.h_hi = (uint32_t)(H >> 64);
+	g.m2 = (uint64_t)p1 * p2; g.h2 = g.m2 / 2;
 	return g;
 }
 

@@ -105,6 +105,14 @@ int ct_prp_check(ct_engine *e, const uint32_t *b, int *prm, uint64_t *res64, int
 int ct_engine_timer_start(ct_engine *e);
 int ct_engine_timer_stop(ct_engine *e, float *ms);
 
+/* Number of NTT primes per squaring.  The engine uses two primes when every
+ * coefficient provably fits (3 n D^2 < p1 p2 / 2, D = max(b, |digits set or
+ * written|)), else three (the reference always uses three, transform.h:26-29);
+ * the carried digits are identical either way.  np = 3 forces three primes,
+ * np = 0 restores the automatic choice. */
+int ct_engine_set_primes(ct_engine *e, int np);
+int ct_engine_primes(const ct_engine *e, int *np);
+
 /* Engine facts: n, lanes, kind actually used. */
 int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind);
 

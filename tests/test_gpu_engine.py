@@ -20,15 +20,23 @@ def oracle_engines(n, bs):
     return [O.Engine(n, int(b), 16) for b in bs]
 
 
+def expected_primes(n, bs, np_mode):
+    D = max(bs)
+    return 3 if np_mode == 3 or 3 * n * D * D >= (2113929217 * 2088763393) // 2 else 2
+
+
+@pytest.mark.parametrize("np_mode", [0, 3])
 @pytest.mark.parametrize("ln", [5, 6, 7, 8, 9, 10, 11, 12, 13, 14])
-def test_square_dup_digit_exact(ln):
+def test_square_dup_digit_exact(ln, np_mode):
     n = 1 << ln
     rng = random.Random(ln)
     lanes = 3
     bs = [rng.randrange(1 << 16, 1 << 22) for _ in range(lanes)]
-    print(f"\n[case] square_dup ln={ln} bs={bs}", flush=True)
+    print(f"\n[case] square_dup ln={ln} np_mode={np_mode} bs={bs}", flush=True)
     t = ct.Transform(ln, lanes, ct.KIND_RESIDENT)
     t.set_b(bs)
+    t.set_primes(np_mode)
+    assert t.primes == expected_primes(n, bs, np_mode)
     x = rand_state(rng, lanes, n, bs)
     t.write_state(0, x)
     oes = oracle_engines(n, bs)
@@ -88,13 +96,14 @@ def test_run_matches_oracle_prp_loop(ln):
         assert O.digits_value(got[l], bs[l]) % N == pow(2, E[l] >> (i0 - steps + 1), N)
 
 
-def test_prp_check_config1():
+@pytest.mark.parametrize("kind", [1, 2])
+def test_prp_check_config1(kind):
     """Full base-2 PRP with Gerbicz-Li at n = 1024, b even in [2^19, 2^20)."""
     n, ln = 1024, 10
     rng = random.Random(20261203)
     bs = [rng.randrange(1 << 18, 1 << 19) * 2 for _ in range(4)]
-    print(f"\n[case] prp_check n=1024 bs={bs}", flush=True)
-    t = ct.Transform(ln, len(bs), ct.KIND_RESIDENT)
+    print(f"\n[case] prp_check n=1024 kind={kind} bs={bs}", flush=True)
+    t = ct.Transform(ln, len(bs), kind)
     prm, r64, gl, res, rc = ct.prp_check(t, bs, want_residue=True)
     assert rc == 0 and gl.all()
     for l, b in enumerate(bs):
@@ -124,16 +133,19 @@ def test_prp_check_known_prp():
 
 # ---------------- plain (multipass) engine ----------------
 
+@pytest.mark.parametrize("np_mode", [0, 3])
 @pytest.mark.parametrize("ln", [10, 11, 12, 13, 14, 16])
-def test_plain_square_dup_digit_exact(ln):
+def test_plain_square_dup_digit_exact(ln, np_mode):
     n = 1 << ln
     rng = random.Random(300 + ln)
     lanes = 2
     bs = [rng.randrange(1 << 16, 1 << 22) for _ in range(lanes)]
-    print(f"\n[case] plain square_dup ln={ln} bs={bs}", flush=True)
+    print(f"\n[case] plain square_dup ln={ln} np_mode={np_mode} bs={bs}", flush=True)
     t = ct.Transform(ln, lanes, ct.KIND_PLAIN)
     assert t.kind == ct.KIND_PLAIN
     t.set_b(bs)
+    t.set_primes(np_mode)
+    assert t.primes == expected_primes(n, bs, np_mode)
     x = rand_state(rng, lanes, n, bs)
     t.write_state(0, x)
     oes = oracle_engines(n, bs)
@@ -147,7 +159,7 @@ def test_plain_square_dup_digit_exact(ln):
     assert np.array_equal(got, ref), f"mismatch at {np.argwhere(got != ref)[:4]}"
 
 
-@pytest.mark.parametrize("ln", [10, 15])
+@pytest.mark.parametrize("ln", [10, 15, 18])
 def test_plain_mul_digit_exact(ln):
     n = 1 << ln
     rng = random.Random(400 + ln)
