@@ -28,7 +28,9 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_square_dup", "ct_engine_mul", "ct_engine_mul_regs", "ct_engine_get_int",
            "ct_engine_is_prime", "ct_engine_read_state", "ct_engine_write_state", "ct_engine_set_exponents",
            "ct_engine_run", "ct_prp_check", "ct_engine_timer_start", "ct_engine_timer_stop", "ct_engine_info",
-           "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes"]
+           "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes",
+           "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
+           "ct_server_stats", "ct_sieve_factors"]
 
 
 class CtError(RuntimeError):
@@ -74,7 +76,15 @@ def lib():
             "ct_engine_timer_stop": [vp, c.POINTER(c.c_float)],
             "ct_engine_info": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(i32)],
             "ct_sieve_window": [i32, i32, u64, u64, u64, u64, u64, u64, i32, u8p],
+            "ct_sieve_factors": [i32, i32, u64, u64, i32, u64p, u64, c.POINTER(u64)],
             "ct_engine_set_primes": [vp, i32],
+            "ct_server_create": [i32, u64, u32, u32, u32, c.POINTER(vp)],
+            "ct_server_destroy": [vp],
+            "ct_server_submit": [vp, u64, u32, u32, u32],
+            "ct_server_step": [vp],
+            "ct_server_poll": [vp, u64p, u64p, u64p, np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"), u32,
+                               c.POINTER(u32)],
+            "ct_server_stats": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u64), c.POINTER(u64)],
             "ct_engine_primes": [vp, c.POINTER(i32)],
         }
         for name, args in sig.items():
@@ -212,6 +222,43 @@ def prp_check(t, bs, want_residue=False):
     return prm.astype(bool), r64, gl.astype(bool), (res.reshape(t.lanes, t.n) if res is not None else None), rc
 
 
+class Server:
+    """Continuous-batching PRP server (config 5): submit(id, log2_n, b, steps),
+    step() rounds, poll() finished (id, res64, hash, latency)."""
+
+    def __init__(self, hbm_budget=8 << 30, slots=64, chunk=250, max_steps=4096, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().ct_server_create(device, int(hbm_budget), slots, chunk, max_steps, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ct_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, rid, log2_n, b, steps):
+        _check(lib().ct_server_submit(self.h, int(rid), int(log2_n), int(b), int(steps)))
+
+    def step(self):
+        _check(lib().ct_server_step(self.h))
+
+    def poll(self, max_results=4096):
+        ids = np.zeros(max_results, np.uint64); r64 = np.zeros(max_results, np.uint64)
+        hs = np.zeros(max_results, np.uint64); lat = np.zeros(max_results, np.float64)
+        cnt = ctypes.c_uint32()
+        _check(lib().ct_server_poll(self.h, ids, r64, hs, lat, max_results, ctypes.byref(cnt)))
+        k = cnt.value
+        return [(int(ids[i]), int(r64[i]), int(hs[i]), float(lat[i])) for i in range(k)]
+
+    def stats(self):
+        q = ctypes.c_uint32(); a = ctypes.c_uint32(); u = ctypes.c_uint64(); s = ctypes.c_uint64()
+        _check(lib().ct_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(u), ctypes.byref(s)))
+        return {"queued": q.value, "active": a.value, "hbm_used": u.value, "squarings": s.value}
+
+
 def res64_string(r):
     """cyclo.h:51-57: uppercase 16-digit hex."""
     return "%016X" % int(r)
@@ -226,6 +273,20 @@ def sieve_window(m, b_min, b_max, p_max_pos, p_max_neg, mode=0, p_min_pos=None, 
         struck = np.zeros(b_max - b_min + 1, np.uint8)
     _check(lib().ct_sieve_window(device, m, b_min, b_max, p_min_pos, p_max_pos, p_min_neg, p_max_neg, mode, struck))
     return struck
+
+
+def sieve_factors(m, p_min, p_max, mode, device=0, cap=1 << 20):
+    """GPU factor emission (ctsieveg): sorted array of (p, b) rows."""
+    while True:
+        pairs = np.zeros(2 * max(cap, 1), np.uint64)
+        cnt = ctypes.c_uint64()
+        rc = lib().ct_sieve_factors(device, m, int(p_min), int(p_max), mode, pairs, cap, ctypes.byref(cnt))
+        if rc == -3 and cnt.value > cap:
+            cap = cnt.value
+            continue
+        _check(rc)
+        out = pairs[:2 * cnt.value].reshape(-1, 2)
+        return out[np.lexsort((out[:, 1], out[:, 0]))]
 
 
 def survivors(struck, b_min):

@@ -247,13 +247,14 @@ static int check_ready(const ct_engine *e)
 	return CT_OK;
 }
 
-int ct::steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words)
+int ct::steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words,
+              const int32_t *lane_i)
 {
 	if (nsteps <= 0) return CT_OK;
-	if (e->kind == CT_KIND_PLAIN) return plain_steps(e, bits, bit_words, i_hi, nsteps, mask_words);
+	if (e->kind == CT_KIND_PLAIN) return plain_steps(e, bits, bit_words, i_hi, nsteps, mask_words, lane_i);
 	res_args A;
 	A.x = e->reg(0); A.tw = e->d_tw; A.dv = e->d_dv;
-	A.bits = bits; A.bit_words = bit_words; A.mask = mask_words;
+	A.bits = bits; A.bit_words = bit_words; A.mask = mask_words; A.lane_i = lane_i;
 	A.i_hi = i_hi; A.nsteps = nsteps; A.gc = e->gc;
 	HIPCHK(resident_steps((int)e->ln, e->np, A, e->lanes, e->st));
 	return CT_OK;

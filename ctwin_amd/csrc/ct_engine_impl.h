@@ -45,14 +45,16 @@ namespace ct {
 int fail(int code, const char *what);
 int hip_fail(hipError_t e, const char *where);
 int canonical(const int32_t *d, uint32_t n, uint32_t b, uint32_t *x);
-int steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words);
+int steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words,
+          const int32_t *lane_i = nullptr);
 int mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
 
 hipError_t launch_set(int32_t *x, uint32_t n, uint32_t lanes, int32_t a, hipStream_t st);
 
 int plain_alloc(ct_engine *e);
 void plain_free(ct_engine *e);
-int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words);
+int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words,
+                const int32_t *lane_i = nullptr);
 int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
 int plain_materialize(ct_engine *e);
 void choose_primes(ct_engine *e);

@@ -1,0 +1,206 @@
+// GPU factor emission (the path of sieve/GPU: kernel.cl:243-479 sequenced by
+// sieve/GPU/src/engine.h:188-220 and gfsieve::check, ctsieve.h:326-431):
+// every pair (p, b) with p a prime of the side's form in [p_min, p_max],
+// 0 <= b < p a root of N+(b) = b^(2^m) - b^(2^(m-1)) + 1 (p = 3k 2^m + 1) or of
+// N-(b) (p = 10k +/- 1) mod p, and b <= bMax(p) (5e9, or 1e10 above 1e16 for
+// + and above 1e13 for -, kernel.cl:304 and :434).  Primality is exact
+// (base-2 strong test, then Miller-Rabin to the first twelve prime bases,
+// deterministic below 3.3e24); the roots come from the same constructions as
+// the window sieve, with 64-bit Montgomery arithmetic.  The host re-checks
+// every pair as saveFactors does (ctsieve.h:270-289).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/ctwin.h"
+#include "ct_engine_impl.h"
+#include "ct_numth.h"
+
+namespace ct {
+
+__device__ __forceinline__ bool is_prime64(uint64_t p)
+{
+	if (p < 2) return false;
+	const uint32_t small[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+	for (int i = 0; i < 12; ++i) { if (p == small[i]) return true; if (p % small[i] == 0) return false; }
+	if (p < 1369) return true;
+	const m64d A(p);
+	if (!mr_prime<m64d, uint64_t>(A, p, small, 1)) return false;        // base 2 first: most composites stop here
+	return mr_prime<m64d, uint64_t>(A, p, small + 1, 11);
+}
+
+__device__ __forceinline__ void fappend(bool keep, uint64_t p, uint64_t *list, uint32_t *count)
+{
+	const uint64_t mask = __ballot(keep);
+	if (mask == 0) return;
+	const int lane = __lane_id();
+	const int leader = __ffsll((long long)mask) - 1;
+	uint32_t base = 0;
+	if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+	base = __shfl(base, leader);
+	if (keep) list[base + __popcll(mask & ((1ull << lane) - 1))] = p;
+}
+
+__device__ __forceinline__ void emit(uint64_t p, uint64_t b, uint64_t bmax, uint64_t *out, unsigned long long *ocount, uint64_t cap)
+{
+	if (b > bmax) return;
+	const unsigned long long slot = atomicAdd(ocount, 1ull);
+	if (slot < cap) { out[2 * slot] = p; out[2 * slot + 1] = b; }
+}
+
+__global__ void k_fprimes(uint64_t k0, uint64_t kcount, int m, int mode, uint64_t p_min, uint64_t p_max,
+                          uint64_t *list, uint32_t *count)
+{
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	for (int s = 0; s < (mode > 0 ? 1 : 2); ++s)
+	{
+		bool keep = false;
+		uint64_t p = 0;
+		if (i < kcount)
+		{
+			const uint64_t k = k0 + i;
+			p = (mode > 0) ? 3 * (k << m) + 1 : 10 * k + 2 * (uint64_t)s - 1;
+			keep = (k > 0 || mode > 0) && p >= p_min && p <= p_max && p >= 11 && is_prime64(p);
+		}
+		fappend(keep, p, list, count);
+	}
+}
+
+// + side: one workgroup per prime
+__global__ void k_froots_pos(const uint64_t *list, const uint32_t *count, int m, uint64_t *out,
+                             unsigned long long *ocount, uint64_t cap)
+{
+	if (blockIdx.x >= *count) return;
+	const uint64_t p = list[blockIdx.x];
+	const m64d A(p);
+	const uint64_t k = (p - 1) / (3ull << m);
+	__shared__ uint64_t sc;
+	if (threadIdx.x == 0)
+	{
+		uint64_t c = 0;
+		for (uint64_t a = 2;; ++a)
+		{
+			c = A.pow(A.to_m(a), k);
+			uint64_t c2 = c;
+			for (int i = 0; i < m - 1; ++i) c2 = A.mul(c2, c2);
+			if (A.mul(A.mul(c2, c2), c2) == A.neg_one() && c2 != A.neg_one()) break;
+		}
+		sc = c;
+	}
+	__syncthreads();
+	const uint64_t c = sc;
+	const uint64_t c2 = A.mul(c, c), c4 = A.mul(c2, c2), c5 = A.mul(c4, c), g = A.mul(c4, c2);
+	const uint64_t bmax = (p < 10000000000000000ull) ? 5000000000ull : 10000000000ull;
+	const uint32_t half = 1u << (m - 1), per = (half + blockDim.x - 1) / blockDim.x;
+	const uint32_t t0 = threadIdx.x * per;
+	if (t0 >= half) return;
+	const uint32_t t1 = min(half, t0 + per);
+	uint64_t cur = A.pow(g, t0);
+	for (uint32_t t = t0; t < t1; ++t)
+	{
+		emit(p, A.from_m(A.mul(c, cur)), bmax, out, ocount, cap);
+		emit(p, A.from_m(A.mul(c5, cur)), bmax, out, ocount, cap);
+		cur = A.mul(cur, g);
+	}
+}
+
+// - side: one thread per prime
+__global__ void k_froots_neg(const uint64_t *list, const uint32_t *count, int m, uint64_t *out,
+                             unsigned long long *ocount, uint64_t cap)
+{
+	const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= *count) return;
+	const uint64_t p = list[idx];
+	const m64d A(p);
+	uint64_t q = p - 1; int v = 0;
+	while ((q & 1) == 0) { q >>= 1; ++v; }
+	const uint64_t w = two_sylow_gen<m64d, uint64_t>(A, p, q);
+	const uint64_t s5 = root_2d<m64d, uint64_t>(A, p, q, v, w, A.to_m(5), 1);
+	if (s5 == 0) return;
+	const uint64_t inv2 = A.to_m((p + 1) / 2);
+	const int d = m - 1, e = (v < d) ? v : d;
+	uint64_t u = w;
+	for (int i = 0; i < v - e; ++i) u = A.mul(u, u);
+	const uint64_t bmax = (p < 10000000000000ull) ? 5000000000ull : 10000000000ull;
+	for (int side = 0; side < 2; ++side)
+	{
+		const uint64_t y = A.mul(side ? A.add(A.one, s5) : A.sub(A.one, s5), inv2);
+		const uint64_t x = root_2d<m64d, uint64_t>(A, p, q, v, w, y, d);
+		if (x == 0) continue;
+		uint64_t r = x;
+		for (uint64_t j = 0; j < (1ull << e); ++j) { emit(p, A.from_m(r), bmax, out, ocount, cap); r = A.mul(r, u); }
+	}
+}
+
+// host re-verification (saveFactors, ctsieve.h:270-289)
+static bool host_divides(uint64_t p, uint64_t b, int m, int mode)
+{
+	const m64h A(p);
+	uint64_t x = b % p;
+	for (int i = 0; i < m - 1; ++i) x = A.mul(x, x);
+	const uint64_t r = A.sub(A.mul(x, x), x);
+	return (mode > 0) ? (r == p - 1) : (r == 1);
+}
+
+}   // namespace ct
+
+using namespace ct;
+
+#define HIPCHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) { rc = hip_fail(_e, #call); goto done; } } while (0)
+
+int ct_sieve_factors(int device, int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap,
+                     uint64_t *count)
+{
+	if (!count || (cap && !pairs)) return fail(CT_ERR_ARG, "bad argument");
+	if (m < 8 || m > 24 || mode == 0) return fail(CT_ERR_ARG, "m in [8, 24], mode +1 or -1");
+	if (p_max < p_min || p_max >= (1ull << 63)) return fail(CT_ERR_UNSUPPORTED, "p range must lie below 2^63");
+	const uint64_t k_lo = (mode > 0) ? (p_min >> m) / 3 : p_min / 10;
+	const uint64_t k_hi = (mode > 0) ? (p_max >> m) / 3 + 1 : p_max / 10 + 1;
+	const uint64_t KB = 1ull << 22;
+	const uint64_t dcap = std::max<uint64_t>(std::min<uint64_t>(cap, 1ull << 24), 1);
+	int rc = CT_OK;
+	uint64_t *d_list = nullptr, *d_out = nullptr;
+	uint32_t *d_count = nullptr;
+	unsigned long long *d_ocount = nullptr;
+	hipStream_t st = nullptr;
+	uint64_t total = 0;
+	{
+		HIPCHK(hipSetDevice(device));
+		HIPCHK(hipStreamCreate(&st));
+		HIPCHK(hipMalloc((void **)&d_list, 2 * KB * sizeof(uint64_t)));
+		HIPCHK(hipMalloc((void **)&d_out, 2 * dcap * sizeof(uint64_t)));
+		HIPCHK(hipMalloc((void **)&d_count, sizeof(uint32_t)));
+		HIPCHK(hipMalloc((void **)&d_ocount, sizeof(unsigned long long)));
+		HIPCHK(hipMemsetAsync(d_ocount, 0, sizeof(unsigned long long), st));
+		for (uint64_t k0 = k_lo; k0 <= k_hi; k0 += KB)
+		{
+			const uint64_t kc = std::min(KB, k_hi - k0 + 1);
+			HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
+			hipLaunchKernelGGL(k_fprimes, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, m, mode, p_min, p_max,
+			                   d_list, d_count);
+			HIPCHK(hipGetLastError());
+			uint32_t cnt = 0;
+			HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+			HIPCHK(hipStreamSynchronize(st));
+			if (!cnt) continue;
+			if (mode > 0) hipLaunchKernelGGL(k_froots_pos, dim3(cnt), dim3(256), 0, st, d_list, d_count, m, d_out, d_ocount, dcap);
+			else hipLaunchKernelGGL(k_froots_neg, dim3((cnt + 127) / 128), dim3(128), 0, st, d_list, d_count, m, d_out, d_ocount, dcap);
+			HIPCHK(hipGetLastError());
+		}
+		unsigned long long oc = 0;
+		HIPCHK(hipMemcpyAsync(&oc, d_ocount, sizeof(oc), hipMemcpyDeviceToHost, st));
+		HIPCHK(hipStreamSynchronize(st));
+		total = oc;
+		*count = total;
+		if (total > cap) { rc = fail(CT_ERR_NOMEM, "pairs buffer too small (count holds the size needed)"); goto done; }
+		if (total > dcap) { rc = fail(CT_ERR_UNSUPPORTED, "more than 2^24 pairs in one call: split the p range"); goto done; }
+		if (total) HIPCHK(hipMemcpy(pairs, d_out, 2 * total * sizeof(uint64_t), hipMemcpyDeviceToHost));
+		for (uint64_t i = 0; i < total; ++i)
+			if (!host_divides(pairs[2 * i], pairs[2 * i + 1], m, mode)) { rc = fail(CT_ERR_COMPUTE, "emitted pair does not divide"); goto done; }
+	}
+done:
+	hipFree(d_list); hipFree(d_out); hipFree(d_count); hipFree(d_ocount);
+	if (st) hipStreamDestroy(st);
+	return rc;
+}

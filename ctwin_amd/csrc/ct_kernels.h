@@ -20,6 +20,8 @@ struct res_args
 	uint32_t bit_words;
 	int i_hi;
 	int nsteps;
+	const int32_t *lane_i;      // per-lane top bit index (continuous batching), or null: i_hi;
+	                            // a negative entry marks an idle lane (its workgroup exits)
 	garner_consts gc;
 };
 

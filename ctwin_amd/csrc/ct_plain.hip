@@ -40,6 +40,8 @@ struct plain_args
 	const uint32_t *mask;       // dup bits when bits is null (or null)
 	uint32_t bit_words;
 	int i;                      // exponent bit index of this squaring
+	const int32_t *lane_i;      // per-lane top bit index (or null); negative = idle lane
+	int step;                   // squarings already done in this run (bit = lane_i - step)
 	uint32_t lanes;
 	garner_consts gc;
 };
@@ -113,6 +115,7 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colA(plain_args A, int 
 	__shared__ int32_t D[G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
 	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
+	if (A.lane_i && A.lane_i[lane] < 0) return;
 	const int32_t *x = (operand ? A.xb : A.x) + (size_t)lane * n;
 	uint32_t *res = (operand ? A.res2 : A.res) + (size_t)lane * n;
 	// chunk role: row r, chunk h of the W columns
@@ -179,6 +182,7 @@ __global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
 	constexpr int CL = LN - RL;
 	__shared__ uint32_t lds[geo<CL>::lds_words];
 	const uint32_t r = blockIdx.x, P = blockIdx.y, lane = blockIdx.z, t = threadIdx.x;
+	if (A.lane_i && A.lane_i[lane] < 0) return;
 	if (P == 0) rowB_prime<LN, RL, 0, MUL>(A, lds, r, lane, t);
 	else if (P == 1) rowB_prime<LN, RL, 1, MUL>(A, lds, r, lane, t);
 	else rowB_prime<LN, RL, 2, MUL>(A, lds, r, lane, t);
@@ -194,6 +198,7 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int 
 	__shared__ uint32_t S[3][G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
 	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
+	if (A.lane_i && A.lane_i[lane] < 0) return;
 	const uint32_t cw = t % G::W, tt = t / G::W;
 	const size_t plane = (size_t)A.lanes * n;
 	const uint32_t *res = A.res + (size_t)lane * n;
@@ -222,7 +227,8 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int 
 	bool dup = false;
 	if (!no_dup)
 	{
-		if (A.bits) dup = (A.bits[(size_t)lane * A.bit_words + (A.i >> 5)] >> (A.i & 31)) & 1u;
+		const int bi = A.lane_i ? A.lane_i[lane] - A.step : A.i;
+		if (A.bits) dup = (A.bits[(size_t)lane * A.bit_words + (bi >> 5)] >> (bi & 31)) & 1u;
 		else if (A.mask) dup = (A.mask[lane >> 5] >> (lane & 31)) & 1u;
 	}
 	int32_t dig[E];
@@ -383,13 +389,14 @@ int plain_materialize(ct_engine *e)
 	return CT_OK;
 }
 
-int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words)
+int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words,
+                const int32_t *lane_i)
 {
 	plain_args A = make_args(e);
-	A.x = e->reg(0); A.bits = bits; A.bit_words = bit_words; A.mask = mask_words;
+	A.x = e->reg(0); A.bits = bits; A.bit_words = bit_words; A.mask = mask_words; A.lane_i = lane_i;
 	for (int s = 0; s < nsteps; ++s)
 	{
-		A.i = i_hi - s;
+		A.i = i_hi - s; A.step = s;
 		A.carry_in = e->pending ? e->d_carry : nullptr;
 		HIPCHK(square_any((int)e->ln, e->np, A, e->lanes, e->st));
 		e->pending = true;

@@ -117,6 +117,8 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
 	__shared__ int32_t D[geo<LN>::lds_words];
 	__shared__ int64_t fbuf[T];
 	const uint32_t c = blockIdx.x, t = threadIdx.x;
+	const int i_top = A.lane_i ? A.lane_i[c] : A.i_hi;
+	if (A.lane_i && i_top < 0) return;          // idle lane (uniform per workgroup)
 	int32_t *x = A.x + (size_t)c * n;
 	{
 		int32_t *dw = D + lbase<LN, 0>(t);
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
 	for (int q = 0; q < E; ++q) none[q] = 0;
 	for (int it = 0; it < A.nsteps; ++it)
 	{
-		const int i = A.i_hi - it;
+		const int i = i_top - it;
 		const bool dup = bits ? ((bits[i >> 5] >> (i & 31)) & 1u) : mdup;
 		step<LN, false, NP>(D, none, dup, dv, lds, fbuf, A.tw, A.gc, t);
 	}

@@ -1,0 +1,329 @@
+// Serving loop: continuous batching of PRP-loop requests of mixed sizes.
+//
+// A request (id, n = 2^ln, b, t) asks for x_t = 2^(top t bits of E) mod N,
+// N = b^n - b^(n/2) + 1, E = N - 1: t squarings of the PRP loop of
+// cyclo.h:277-308 from x = 1.  Requests are grouped by n into size classes;
+// each class owns one engine with `slots` lanes (resident kernels for
+// n <= 2^14, the multipass engine above) and its own HIP stream.  A request is
+// admitted into a free lane of its class as soon as one frees up (its digits,
+// b, exponent window and bit offset are written for that lane only; the other
+// lanes keep running), and a new class engine is created only while the sum
+// of the engines' device allocations stays within the HBM budget; idle classes
+// are released when a queued request needs the memory.  Each round launches
+// min(chunk, shortest remaining) squarings on every busy class, all streams
+// first, then waits, so the classes' kernels overlap on the device.
+#include <hip/hip_runtime.h>
+#include <gmp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/ctwin.h"
+#include "ct_engine_impl.h"
+
+using namespace ct;
+
+namespace {
+
+double now_s()
+{
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct request { uint64_t id; uint32_t ln, b, steps; double t_submit; };
+
+struct lane_state { bool busy = false; request req{}; int remaining = 0; };
+
+struct size_class
+{
+	ct_engine *e = nullptr;
+	uint32_t ln = 0, words = 0;
+	int32_t *d_lane_i = nullptr;
+	std::vector<int32_t> lane_i;
+	std::vector<lane_state> lanes;
+	std::vector<uint32_t> stage_bits;
+	std::vector<int32_t> stage_digits;
+	uint64_t bytes = 0;
+	bool launched = false;
+};
+
+struct result { uint64_t id, res64, hash; double latency; };
+
+// Top k bits of E = b^n - b^(n/2): w = E >> s (certified truncated squaring of
+// b; exact arithmetic when E is short).  Returns false when not certified.
+bool exponent_window(uint32_t b, uint32_t n, uint32_t k, mpz_t w)
+{
+	const uint64_t ebits = (uint64_t)n * 32;
+	if (ebits < 4 * (uint64_t)k + 4096)
+	{
+		mpz_t h, E; mpz_inits(h, E, NULL);
+		mpz_ui_pow_ui(h, b, n / 2);
+		mpz_mul(E, h, h); mpz_sub(E, E, h);
+		const size_t len = mpz_sizeinbase(E, 2);
+		if (len > k) mpz_fdiv_q_2exp(w, E, len - k); else mpz_set(w, E);
+		mpz_clears(h, E, NULL);
+		return true;
+	}
+	const size_t P = k + 96;
+	mpz_t lo, hi, t; mpz_inits(lo, hi, t, NULL);
+	mpz_set_ui(lo, b); mpz_set_ui(hi, b);
+	uint64_t e = 0;
+	for (uint32_t m = n; m > 1; m >>= 1)
+	{
+		mpz_mul(lo, lo, lo); mpz_mul(hi, hi, hi); e *= 2;
+		const size_t hb = mpz_sizeinbase(hi, 2);
+		if (hb > P)
+		{
+			const size_t sh = hb - P;
+			mpz_fdiv_q_2exp(lo, lo, sh); mpz_fdiv_q_2exp(hi, hi, sh); mpz_add_ui(hi, hi, 1);
+			e += sh;
+		}
+	}
+	bool ok = mpz_sizeinbase(lo, 2) == mpz_sizeinbase(hi, 2);
+	const size_t sl = mpz_sizeinbase(hi, 2) - k;
+	mpz_fdiv_q_2exp(w, lo, sl); mpz_fdiv_q_2exp(t, hi, sl);
+	ok = ok && mpz_cmp(w, t) == 0;
+	const uint64_t bl = 64 - __builtin_clzll((uint64_t)b);
+	mpz_fdiv_r_2exp(t, lo, sl);
+	ok = ok && (e + sl > (uint64_t)(n / 2) * bl + 64) && mpz_sgn(t) != 0;
+	mpz_clears(lo, hi, t, NULL);
+	return ok;
+}
+
+uint64_t class_bytes(uint32_t ln, uint32_t slots, uint32_t words)
+{
+	const uint64_t n = 1ull << ln;
+	uint64_t b = 3 * (uint64_t)slots * n * 4 + 48 * n + (uint64_t)slots * (words * 4 + 64);
+	if (ln > 14) b += 2 * 3 * (uint64_t)slots * n * 4 + (uint64_t)slots * (n / 16) * 8;
+	return b;
+}
+
+}   // namespace
+
+struct ct_server
+{
+	int device = 0;
+	uint64_t budget = 0, used = 0;
+	uint32_t slots = 0, chunk = 0, max_steps = 0;
+	std::deque<request> queue;
+	std::map<uint32_t, std::unique_ptr<size_class>> classes;
+	std::vector<result> done;
+	uint64_t squarings = 0;
+};
+
+static void destroy_class(ct_server *s, uint32_t ln)
+{
+	auto it = s->classes.find(ln);
+	if (it == s->classes.end()) return;
+	size_class &c = *it->second;
+	hipFree(c.d_lane_i);
+	ct_engine_destroy(c.e);
+	s->used -= c.bytes;
+	s->classes.erase(it);
+}
+
+int ct_server_create(int device, uint64_t hbm_budget, uint32_t slots, uint32_t chunk, uint32_t max_steps, ct_server **out)
+{
+	if (!out || slots == 0 || chunk == 0 || max_steps == 0) return fail(CT_ERR_ARG, "bad argument");
+	ct_server *s = new ct_server();
+	s->device = device; s->budget = hbm_budget; s->slots = slots; s->chunk = chunk; s->max_steps = max_steps;
+	*out = s;
+	return CT_OK;
+}
+
+int ct_server_destroy(ct_server *s)
+{
+	if (!s) return CT_OK;
+	while (!s->classes.empty()) destroy_class(s, s->classes.begin()->first);
+	delete s;
+	return CT_OK;
+}
+
+int ct_server_submit(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps)
+{
+	if (!s) return fail(CT_ERR_ARG, "null server");
+	if (log2_n < 10 || log2_n > 22) return fail(CT_ERR_UNSUPPORTED, "served sizes are 2^10 .. 2^22");
+	if (b <= 32 || b >= 0x80000000u) return fail(CT_ERR_ARG, "b must satisfy 32 < b < 2^31");
+	if (steps == 0 || steps > s->max_steps) return fail(CT_ERR_ARG, "steps out of range");
+	s->queue.push_back(request{id, log2_n, b, steps, now_s()});
+	return CT_OK;
+}
+
+static int make_class(ct_server *s, uint32_t ln, size_class **out)
+{
+	const uint32_t words = (s->max_steps + 31) / 32 + 1;
+	const uint64_t bytes = class_bytes(ln, s->slots, words);
+	if (s->used + bytes > s->budget) return CT_ERR_NOMEM;
+	std::unique_ptr<size_class> c(new size_class());
+	int rc = ct_engine_create(s->device, ln, s->slots, 16, CT_KIND_AUTO, &c->e);
+	if (rc) return rc;
+	c->ln = ln; c->words = words; c->bytes = bytes;
+	c->lanes.resize(s->slots);
+	c->lane_i.assign(s->slots, -1);
+	// every lane gets a valid b before the first launch (idle lanes never run)
+	std::vector<uint32_t> b0(s->slots, 33);
+	if ((rc = ct_engine_set_b(c->e, b0.data()))) { ct_engine_destroy(c->e); return rc; }
+	c->e->b_max = 0; c->e->state_bound = 1;
+	std::vector<uint32_t> zeros((size_t)s->slots * words, 0);
+	if ((rc = ct_engine_set_exponents(c->e, zeros.data(), words))) { ct_engine_destroy(c->e); return rc; }
+	if (hipMalloc((void **)&c->d_lane_i, s->slots * sizeof(int32_t)) != hipSuccess)
+	{ ct_engine_destroy(c->e); return fail(CT_ERR_NOMEM, "hipMalloc lane offsets"); }
+	c->stage_bits.assign((size_t)s->slots * words, 0);
+	c->stage_digits.assign((size_t)s->slots * c->e->n, 0);
+	s->used += bytes;
+	*out = c.get();
+	s->classes[ln] = std::move(c);
+	return CT_OK;
+}
+
+// Writes request r into lane l of class c (stream-ordered on the class stream).
+static int admit(size_class &c, uint32_t l, const request &r)
+{
+	ct_engine *e = c.e;
+	const uint32_t n = e->n;
+	mpz_t w; mpz_init(w);
+	if (!exponent_window(r.b, n, r.steps, w)) { mpz_clear(w); return fail(CT_ERR_COMPUTE, "exponent window not certified"); }
+	uint32_t *bits = &c.stage_bits[(size_t)l * c.words];
+	std::memset(bits, 0, c.words * sizeof(uint32_t));
+	size_t cnt = 0;
+	mpz_export(bits, &cnt, -1, sizeof(uint32_t), 0, 0, w);
+	mpz_clear(w);
+	int32_t *dig = &c.stage_digits[(size_t)l * n];
+	std::memset(dig, 0, n * sizeof(int32_t)); dig[0] = 1;
+	e->b[l] = r.b;
+	e->b_max = std::max<uint64_t>(e->b_max, r.b);
+	choose_primes(e);
+	const divisor dv = make_divisor(r.b);
+	hipMemcpyAsync(e->d_dv + l, &dv, sizeof(divisor), hipMemcpyHostToDevice, e->st);
+	hipMemcpyAsync(e->d_bits + (size_t)l * c.words, bits, c.words * sizeof(uint32_t), hipMemcpyHostToDevice, e->st);
+	hipMemcpyAsync(e->reg(0) + (size_t)l * n, dig, n * sizeof(int32_t), hipMemcpyHostToDevice, e->st);
+	if (e->kind == CT_KIND_PLAIN) hipMemsetAsync(e->d_carry + (size_t)l * (n / 16), 0, (n / 16) * sizeof(int64_t), e->st);
+	hipStreamSynchronize(e->st);      // divisor staging lives on this stack frame
+	c.lanes[l].busy = true; c.lanes[l].req = r; c.lanes[l].remaining = (int)r.steps;
+	c.lane_i[l] = (int32_t)r.steps - 1;
+	return CT_OK;
+}
+
+int ct_server_step(ct_server *s)
+{
+	if (!s) return fail(CT_ERR_ARG, "null server");
+	int rc = hipSetDevice(s->device) == hipSuccess ? CT_OK : fail(CT_ERR_HIP, "hipSetDevice");
+	if (rc) return rc;
+	// 1. admission: FIFO, a request may overtake one whose class is full or
+	//    whose class cannot be allocated within the budget
+	for (auto it = s->queue.begin(); it != s->queue.end();)
+	{
+		const request r = *it;
+		auto ci = s->classes.find(r.ln);
+		size_class *c = (ci != s->classes.end()) ? ci->second.get() : nullptr;
+		if (!c)
+		{
+			rc = make_class(s, r.ln, &c);
+			if (rc == CT_ERR_NOMEM)
+			{   // release idle classes and retry once
+				std::vector<uint32_t> idle;
+				for (auto &kv : s->classes)
+				{
+					bool any = false;
+					for (auto &ls : kv.second->lanes) any |= ls.busy;
+					if (!any) idle.push_back(kv.first);
+				}
+				for (uint32_t ln : idle) destroy_class(s, ln);
+				rc = make_class(s, r.ln, &c);
+			}
+			if (rc == CT_ERR_NOMEM)
+			{
+				if (s->classes.empty()) return fail(CT_ERR_NOMEM, "HBM budget below one size class");
+				++it; continue;
+			}
+			if (rc) return rc;
+		}
+		uint32_t l = 0;
+		while (l < c->lanes.size() && c->lanes[l].busy) ++l;
+		if (l == c->lanes.size()) { ++it; continue; }
+		if ((rc = admit(*c, l, r))) return rc;
+		it = s->queue.erase(it);
+	}
+	// 2. launch a chunk on every busy class (all streams before any wait)
+	for (auto &kv : s->classes)
+	{
+		size_class &c = *kv.second;
+		c.launched = false;
+		int k = (int)s->chunk;
+		bool any = false;
+		for (auto &ls : c.lanes) if (ls.busy) { any = true; k = std::min(k, ls.remaining); }
+		if (!any) continue;
+		ct_engine *e = c.e;
+		hipMemcpyAsync(c.d_lane_i, c.lane_i.data(), c.lane_i.size() * sizeof(int32_t), hipMemcpyHostToDevice, e->st);
+		if ((rc = steps(e, e->d_bits, c.words, 0, k, nullptr, c.d_lane_i))) return rc;
+		for (uint32_t l = 0; l < c.lanes.size(); ++l)
+			if (c.lanes[l].busy) { c.lanes[l].remaining -= k; c.lane_i[l] -= k; s->squarings += k; }
+		c.launched = true;
+	}
+	// 3. wait, read out finished lanes
+	for (auto &kv : s->classes)
+	{
+		size_class &c = *kv.second;
+		if (!c.launched) continue;
+		ct_engine *e = c.e;
+		if (hipStreamSynchronize(e->st) != hipSuccess) return fail(CT_ERR_HIP, "class stream failed");
+		bool finished = false;
+		for (auto &ls : c.lanes) finished |= (ls.busy && ls.remaining == 0);
+		if (!finished) continue;
+		if (e->kind == CT_KIND_PLAIN && (rc = plain_materialize(e))) return rc;
+		hipStreamSynchronize(e->st);
+		const uint32_t n = e->n;
+		std::vector<int32_t> raw(n);
+		std::vector<uint32_t> canon(n);
+		for (uint32_t l = 0; l < c.lanes.size(); ++l)
+		{
+			lane_state &ls = c.lanes[l];
+			if (!(ls.busy && ls.remaining == 0)) continue;
+			if (hipMemcpy(raw.data(), e->reg(0) + (size_t)l * n, n * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+				return fail(CT_ERR_HIP, "read-out");
+			if (canonical(raw.data(), n, ls.req.b, canon.data())) return fail(CT_ERR_GETINT, "getInt failed");
+			uint64_t r64 = canon[0], bi = ls.req.b, h = 1469598103934665603ull;
+			for (uint32_t j = 1; j < n; ++j) { r64 += (uint64_t)canon[j] * bi; bi *= ls.req.b; }
+			for (uint32_t j = 0; j < n; ++j) { h ^= canon[j]; h *= 1099511628211ull; }
+			s->done.push_back(result{ls.req.id, r64, h, now_s() - ls.req.t_submit});
+			ls.busy = false;
+			c.lane_i[l] = -1;
+		}
+	}
+	return CT_OK;
+}
+
+int ct_server_poll(ct_server *s, uint64_t *ids, uint64_t *res64, uint64_t *hash, double *latency_s, uint32_t max,
+                   uint32_t *count)
+{
+	if (!s || !count) return fail(CT_ERR_ARG, "bad argument");
+	const uint32_t k = (uint32_t)std::min<size_t>(max, s->done.size());
+	for (uint32_t i = 0; i < k; ++i)
+	{
+		const result &r = s->done[i];
+		if (ids) ids[i] = r.id;
+		if (res64) res64[i] = r.res64;
+		if (hash) hash[i] = r.hash;
+		if (latency_s) latency_s[i] = r.latency;
+	}
+	s->done.erase(s->done.begin(), s->done.begin() + k);
+	*count = k;
+	return CT_OK;
+}
+
+int ct_server_stats(const ct_server *s, uint32_t *queued, uint32_t *active, uint64_t *hbm_used, uint64_t *squarings)
+{
+	if (!s) return fail(CT_ERR_ARG, "null server");
+	uint32_t a = 0;
+	for (auto &kv : s->classes) for (auto &ls : kv.second->lanes) a += ls.busy ? 1 : 0;
+	if (queued) *queued = (uint32_t)s->queue.size();
+	if (active) *active = a;
+	if (hbm_used) *hbm_used = s->used;
+	if (squarings) *squarings = s->squarings;
+	return CT_OK;
+}

@@ -116,6 +116,30 @@ int ct_engine_primes(const ct_engine *e, int *np);
 /* Engine facts: n, lanes, kind actually used. */
 int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind);
 
+/* ---------------- serving loop ---------------- */
+
+/* Continuous batching of PRP-loop requests of mixed sizes.  Replaces the
+ * batch loop of cyclo::check(filename) (cyclo.h:414-485), which groups the
+ * candidates of one n into fixed 64-lane batches (parseFile, cyclo.h:216-241)
+ * and runs one batch at a time.  A request (id, 2^log2_n digits, b, steps)
+ * yields x = 2^(top `steps` bits of N - 1) mod N, N = b^n - b^(n/2) + 1.
+ * Requests are admitted into free lanes of per-size engines as lanes free up;
+ * a new size class is created only within hbm_budget bytes of device memory.
+ * slots = lanes per size class; chunk = squarings per scheduling round;
+ * max_steps bounds `steps`.  10 <= log2_n <= 22. */
+typedef struct ct_server ct_server;
+int ct_server_create(int device, uint64_t hbm_budget, uint32_t slots, uint32_t chunk, uint32_t max_steps, ct_server **out);
+int ct_server_destroy(ct_server *s);
+int ct_server_submit(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps);
+/* One round: admit what fits, launch a chunk on every busy size class (their
+ * streams overlap), wait, read out finished requests. */
+int ct_server_step(ct_server *s);
+/* Finished requests since the last poll: id, RES64 of the canonical residue,
+ * a 64-bit FNV-1a hash of its digits, latency from submit (s). */
+int ct_server_poll(ct_server *s, uint64_t *ids, uint64_t *res64, uint64_t *hash, double *latency_s, uint32_t max,
+                   uint32_t *count);
+int ct_server_stats(const ct_server *s, uint32_t *queued, uint32_t *active, uint64_t *hbm_used, uint64_t *squarings);
+
 /* ---------------- window sieve ---------------- */
 
 /* Sieve::check_pos / check_neg (sieve/CPU/ctsieve.cpp:678-787) on the GPU:
@@ -127,6 +151,19 @@ int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind);
  * unstruck b in ascending order (cand_*.txt of ctsieve.cpp:621-626). */
 int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_max_pos,
                     uint64_t p_min_neg, uint64_t p_max_neg, int mode, uint8_t *struck);
+
+/* GPU factor emission: the work of `ctsieveg <n> <p_min> <p_max> <+|->`
+ * (sieve/GPU/src/main.cpp:113-169, gfsieve::check, ctsieve.h:326-431, kernels
+ * sieve/GPU/ocl/kernel.cl:243-479).  Every pair (p, b) with p prime in
+ * [p_min, p_max] of the form 3k 2^m + 1 (mode +1) or 10k +/- 1 (mode -1),
+ * 0 <= b < p a root of b^(2^m) - b^(2^(m-1)) +/- 1 mod p and b <= bMax(p) (5e9;
+ * 1e10 above 1e16 (+) or 1e13 (-)), every pair re-verified on the host as in
+ * saveFactors (ctsieve.h:270-289).  pairs[2i] = p, pairs[2i+1] = b, unordered
+ * (the reference appends with atomics too); *count = number of pairs;
+ * CT_ERR_NOMEM when it exceeds cap (count then holds the size needed).
+ * p_max < 2^63. */
+int ct_sieve_factors(int device, int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap,
+                     uint64_t *count);
 
 /* Last error text of this thread (static storage). */
 const char *ct_last_error(void);

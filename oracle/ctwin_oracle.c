@@ -940,3 +940,192 @@ int or_sieve_window(int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, u
 	if (mode <= 0 && check_neg(m, b_min, b_max, p_min_neg, p_max_neg, struck) < 0) return -1;
 	return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* GPU factor emission: sieve/GPU/ocl/kernel.cl, sieve/GPU/src/ctsieve.h */
+/* ------------------------------------------------------------------ */
+
+/* kernel.cl:88-100: base-2 Euler test in Montgomery form */
+static int fe_prp(const mpa *m) { return mp_prp(m); }
+
+/* kernel.cl:103-128 */
+static int fe_jacobi(uint64_t x, uint64_t y) { return or_jacobi(x, y); }
+
+/* kernel.cl:131-140 (p = 3 mod 4), :142-166 (p = 5 mod 8 Atkin), Cipolla :142-166 */
+static uint64_t fe_sqrt_3_4(const mpa *m, uint64_t a) { return mp_pow(m, a, (m->p + 1) / 4); }
+static uint64_t fe_sqrt_5_8(const mpa *m, uint64_t a)
+{
+	const uint64_t b = mp_add(m, a, a);
+	const uint64_t v = mp_pow(m, b, (m->p - 5) / 8);
+	const uint64_t i = mp_mul(m, b, mp_mul(m, v, v));
+	return mp_mul(m, mp_mul(m, a, v), mp_sub(m, i, m->one));
+}
+static uint64_t fe_sqrt_cipolla(const mpa *m, uint64_t a)
+{
+	const uint64_t p = m->p;
+	if ((a == 0) || (a == m->one)) return a;
+	uint32_t d = 2; uint64_t md = mp_add(m, m->one, m->one), ai = p - mp_toint(m, a);
+	while (d < 256)
+	{
+		if (fe_jacobi((uint64_t)d * d + ai, p) == -1) break;
+		++d; md = mp_add(m, md, m->one);
+	}
+	if (d >= 256) return 0;
+	const uint64_t w2 = mp_sub(m, mp_mul(m, md, md), a), e = (p + 1) / 2;
+	uint64_t x = md, y = m->one;
+	for (int b = ilog2_64(e) - 1; b >= 0; --b)
+	{
+		const uint64_t t = mp_add(m, mp_mul(m, x, x), mp_mul(m, mp_mul(m, y, y), w2));
+		y = mp_mul(m, mp_add(m, x, x), y); x = t;
+		if ((e & ((uint64_t)1 << b)) != 0)
+		{
+			const uint64_t t2 = mp_add(m, mp_mul(m, x, md), mp_mul(m, y, w2));
+			y = mp_add(m, x, mp_mul(m, y, md)); x = t2;
+		}
+	}
+	return (mp_mul(m, x, x) == a) ? x : 0;
+}
+
+/* kernel.cl:168-231 sqrtn variants: sel = 0 (3 mod 4), 1 (5 mod 8), 2 (Cipolla) */
+static uint64_t fe_sqrtn(const mpa *m, uint64_t a, uint32_t n, int sel)
+{
+	const uint64_t p = m->p;
+	uint64_t r = a, k = (p - 1) / 2, mm = 1;
+	if (sel == 0)
+	{
+		for (uint32_t i = 1; i <= n; ++i) { if (mm % 2 != 0) mm += k; mm /= 2; }
+		if (mm > 1) r = mp_pow(m, r, mm);
+		return r;
+	}
+	uint32_t e = 1; for (uint64_t q = (p - 1) / 2; q % 2 == 0; q /= 2) ++e;
+	if (n < e) e = n;
+	if ((e > 1) && mp_pow(m, a, (p - 1) >> e) != m->one) return 0;
+	for (uint32_t i = 1; i <= n; ++i)
+	{
+		if (k % 2 == 0)
+		{
+			r = (sel == 1) ? fe_sqrt_5_8(m, r) : fe_sqrt_cipolla(m, r);
+			k /= 2;
+			if (r == 0) break;
+		}
+		else { if (mm % 2 != 0) mm += k; mm /= 2; }
+	}
+	if (mm > 1) r = mp_pow(m, r, mm);
+	return r;
+}
+
+typedef struct { uint64_t *pairs; uint64_t cap, count; int m; int mode; int err; } fe_out;
+
+/* kernel.cl:233-241 push_factor + ctsieve.h:270-289 host verification */
+static void fe_push(fe_out *o, uint64_t p, uint64_t b, uint64_t bmax)
+{
+	if (b > bmax) return;
+	const uint64_t x = md_pow(b, (uint64_t)1 << (o->m - 1), p), r = md_sub(md_mul(x, x, p), x, p);
+	if (!(((o->mode > 0) && (r == p - 1)) || ((o->mode < 0) && (r == 1))))
+	{
+		if (or_isprime64(p)) o->err = 1;
+		return;
+	}
+	if (o->count < o->cap) { o->pairs[2 * o->count] = p; o->pairs[2 * o->count + 1] = b; }
+	++o->count;
+}
+
+int or_sieve_factors(int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap, uint64_t *count)
+{
+	fe_out o = { pairs, cap, 0, m, mode, 0 };
+	if (mode > 0)
+	{
+		for (uint64_t k = (p_min >> m) / 3; ; ++k)
+		{
+			const uint64_t p = 3 * (k << m) + 1;
+			if (p > p_max) break;
+			if (p < p_min) continue;
+			mpa M; mp_init(&M, p);
+			if (!fe_prp(&M)) continue;
+			/* init_factors_pos, kernel.cl:258-288: odd a >= 5, a != 0 mod 3 */
+			const uint64_t one = M.one, two = mp_add(&M, one, one), four = mp_add(&M, two, two);
+			const uint64_t pm1_6 = (p - 1) / 6;
+			uint32_t a = 5; uint64_t ma = mp_add(&M, four, one);
+			while (a < 256)
+			{
+				if ((fe_jacobi(p % a, a) == -1) && (mp_pow(&M, ma, pm1_6) != p - one)) break;
+				a += 2; ma = mp_add(&M, ma, two);
+				if ((fe_jacobi(p % a, a) == -1) && (mp_pow(&M, ma, pm1_6) != p - one)) break;
+				a += 4; ma = mp_add(&M, ma, four);
+			}
+			if (a >= 256) continue;
+			uint64_t b = mp_pow(&M, ma, pm1_6 >> (m - 1));
+			/* generate_factors_pos, kernel.cl:290-320 */
+			const uint64_t b2 = mp_mul(&M, b, b), b4 = mp_mul(&M, b2, b2);
+			b = mp_toint(&M, b);
+			const uint64_t bmax = (p < 10000000000000000ull) ? 5000000000ull : 10000000000ull;
+			for (uint32_t j = 1; j < (3u << (m - 1)); j += 6)
+			{
+				fe_push(&o, p, b, bmax); fe_push(&o, p, p - b, bmax);
+				b = mp_mul(&M, b, b4);
+				fe_push(&o, p, b, bmax); fe_push(&o, p, p - b, bmax);
+				b = mp_mul(&M, b, b2);
+			}
+		}
+	}
+	else
+	{
+		for (uint64_t k = p_min / 10; ; ++k)
+		{
+			if (10 * k - 1 > p_max && k > 0) break;
+			for (int i = 0; i <= 1; ++i)
+			{
+				if (k == 0 && i == 0) continue;
+				const uint64_t p = 10 * k + 2 * (uint64_t)i - 1;
+				if (p < p_min || p > p_max || p < 11) continue;
+				mpa M; mp_init(&M, p);
+				if (!fe_prp(&M)) continue;
+				const uint64_t one = M.one, two = mp_add(&M, one, one), five = mp_add(&M, mp_add(&M, two, two), one);
+				uint64_t r1 = 0, r2 = 0;
+				if (p % 4 == 3)
+				{   /* init_3_4_factors_neg, kernel.cl:363-381 */
+					const uint64_t s5 = fe_sqrt_3_4(&M, five);
+					if (s5 == 0) continue;
+					const uint64_t r = mp_half(&M, mp_sub(&M, one, s5));
+					const uint64_t rs = (fe_jacobi(mp_toint(&M, r), p) == 1) ? r : mp_sub(&M, one, r);
+					r1 = fe_sqrtn(&M, rs, m - 1, 0);
+				}
+				else
+				{   /* init_5_8_factors_neg :383-401, init_factors_neg :403-421 */
+					const int sel = (p % 8 == 5) ? 1 : 2;
+					const uint64_t s5 = (sel == 1) ? fe_sqrt_5_8(&M, five) : fe_sqrt_cipolla(&M, five);
+					if (s5 == 0) continue;
+					const uint64_t r = mp_half(&M, mp_sub(&M, one, s5));
+					if (fe_jacobi(mp_toint(&M, r), p) != 1) continue;
+					r1 = fe_sqrtn(&M, r, m - 1, sel);
+					r2 = fe_sqrtn(&M, mp_sub(&M, one, r), m - 1, sel);
+				}
+				/* generate_factors_neg, kernel.cl:423-471 */
+				const uint64_t bmax = (p < 10000000000000ull) ? 5000000000ull : 10000000000ull;
+				uint32_t e = 1; for (uint64_t q = (p - 1) / 2; q % 2 == 0; q /= 2) ++e;
+				if (e > (uint32_t)(m - 1)) e = (uint32_t)(m - 1);
+				uint32_t d = 2; uint64_t md = two;
+				while (d < 256)
+				{
+					if (fe_jacobi(d, p) == -1) break;
+					++d; md = mp_add(&M, md, one); if ((d == 4) || (d == 9)) { ++d; md = mp_add(&M, md, one); }
+				}
+				if (d >= 256) continue;
+				const uint64_t u = mp_pow(&M, md, (p - 1) >> e);
+				for (int w = 0; w < 2; ++w)
+				{
+					const uint64_t rr = w ? r2 : r1;
+					if (rr == 0) continue;
+					uint64_t b = mp_toint(&M, rr);
+					for (uint32_t t = 0, s = 1u << e; t < s; t += 2)
+					{
+						fe_push(&o, p, b, bmax); fe_push(&o, p, p - b, bmax);
+						b = mp_mul(&M, b, u);
+					}
+				}
+			}
+		}
+	}
+	*count = o.count;
+	return o.err ? -1 : 0;
+}

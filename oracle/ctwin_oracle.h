@@ -130,6 +130,17 @@ int or_isprime64(uint64_t n);
 int or_sieve_window(int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_max_pos,
                     uint64_t p_min_neg, uint64_t p_max_neg, int mode, uint8_t *struck);
 
+/* GPU factor emission of sieve/GPU (kernel.cl:243-479 + ctsieve.h:248-297)
+ * restated on the CPU: for p in [p_min, p_max] of the side's form that pass
+ * the base-2 Euler test, the root seeds of init_factors_* (a-search with the
+ * Kronecker symbol, giving up at a >= 256; Atkin / Cipolla square roots), the
+ * root enumeration of generate_factors_* with b <= bMax (5e9, or 1e10 above
+ * 1e16 (+) / 1e13 (-)), and the host re-verification of saveFactors (pairs
+ * that fail are dropped when p is composite; -1 is returned when p is prime).
+ * Pairs are written as pairs[2i] = p, pairs[2i+1] = b, in emission order;
+ * *count gets the total even when it exceeds cap. */
+int or_sieve_factors(int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap, uint64_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,8 @@ def lib():
         L.or_sieve_window.argtypes = [c.c_int, c.c_uint64, c.c_uint64, c.c_uint64, c.c_uint64,
                                       c.c_uint64, c.c_uint64, c.c_int, u8p]
         L.or_sieve_window.restype = c.c_int
+        L.or_sieve_factors.argtypes = [c.c_int, c.c_uint64, c.c_uint64, c.c_int, u64p, c.c_uint64, c.POINTER(c.c_uint64)]
+        L.or_sieve_factors.restype = c.c_int
         _lib = L
     return _lib
 
@@ -168,6 +170,16 @@ def sieve_window(m, b_min, b_max, p_max_pos, p_max_neg, mode=0, p_min_pos=None, 
     rc = lib().or_sieve_window(m, b_min, b_max, p_min_pos, p_max_pos, p_min_neg, p_max_neg, mode, struck)
     assert rc == 0
     return struck
+
+
+def sieve_factors(m, p_min, p_max, mode, cap=1 << 20):
+    """Oracle factor emission: sorted array of (p, b) rows."""
+    pairs = np.zeros(2 * cap, np.uint64)
+    cnt = ctypes.c_uint64()
+    rc = lib().or_sieve_factors(m, p_min, p_max, mode, pairs, cap, ctypes.byref(cnt))
+    assert rc == 0 and cnt.value <= cap
+    out = pairs[:2 * cnt.value].reshape(-1, 2)
+    return out[np.lexsort((out[:, 1], out[:, 0]))]
 
 
 # ---- pure-Python helpers (independent of the oracle) ----

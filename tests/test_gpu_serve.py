@@ -1,0 +1,99 @@
+"""Serving loop (config 5): continuous batching of mixed sizes.  Every result
+must equal the same request run alone, and Python big-integer pow for the
+sizes where that is quick."""
+import random
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ct = pytest.importorskip("ctwin_amd")
+
+
+def fnv(canon):
+    h = 1469598103934665603
+    for d in canon:
+        h ^= d
+        h = (h * 1099511628211) % (1 << 64)
+    return h
+
+
+def python_result(ln, b, steps):
+    n = 1 << ln
+    E = b ** n - b ** (n // 2)
+    N = E + 1
+    x = pow(2, E >> (E.bit_length() - steps), N)
+    canon = []
+    v = x
+    for _ in range(n):
+        v, r = divmod(v, b)
+        canon.append(r)
+    return x % (1 << 64), fnv(canon)
+
+
+def drain(s, want):
+    out = {}
+    guard = 0
+    while len(out) < want:
+        s.step()
+        for rid, r64, h, lat in s.poll():
+            out[rid] = (r64, h, lat)
+        guard += 1
+        assert guard < 100000
+    return out
+
+
+def make_requests(seed, count, sizes, steps):
+    rng = random.Random(seed)
+    reqs = []
+    for i in range(count):
+        ln = rng.choice(sizes)
+        bits = rng.randrange(16, 23)
+        b = rng.randrange(1 << (bits - 1), 1 << bits)
+        reqs.append((i, ln, b, steps))
+    return reqs
+
+
+def test_serve_mixed_equals_alone_and_python():
+    reqs = make_requests(5, 36, [10, 11, 12, 13, 14, 15, 16], 300)
+    print(f"\n[case] serve mixed: {len(reqs)} requests", flush=True)
+    s = ct.Server(hbm_budget=4 << 30, slots=4, chunk=64, max_steps=1024)
+    # bursts: submit a few, run a round, submit more while others are in flight
+    it = iter(reqs)
+    mixed = {}
+    pending = list(reqs)
+    while pending:
+        for _ in range(5):
+            if pending:
+                s.submit(*pending.pop(0))
+        s.step()
+        for rid, r64, h, lat in s.poll():
+            mixed[rid] = (r64, h)
+    for rid, v in drain(s, len(reqs) - len(mixed)).items():
+        mixed[rid] = v[:2]
+    assert len(mixed) == len(reqs)
+    s.close()
+    alone = ct.Server(hbm_budget=4 << 30, slots=1, chunk=64, max_steps=1024)
+    for r in reqs:
+        alone.submit(*r)
+        (rid, v), = drain(alone, 1).items()
+        assert rid == r[0]
+        assert mixed[rid] == v[:2], f"request {r} differs alone vs mixed"
+    for rid, ln, b, steps in reqs:
+        if ln <= 12:
+            assert mixed[rid] == python_result(ln, b, steps), f"request {(rid, ln, b)} vs Python pow"
+
+
+def test_serve_budget_forces_class_recycling():
+    reqs = make_requests(9, 12, [14, 16, 18], 200)
+    # budget fits one class of 2^18 x 2 slots but not three classes at once
+    s = ct.Server(hbm_budget=48 << 20, slots=2, chunk=50, max_steps=512)
+    for r in reqs:
+        s.submit(*r)
+    got = drain(s, len(reqs))
+    st = s.stats()
+    assert st["hbm_used"] <= 48 << 20 and st["queued"] == 0 and st["active"] == 0
+    alone = ct.Server(hbm_budget=1 << 30, slots=1, chunk=50, max_steps=512)
+    for r in reqs:
+        alone.submit(*r)
+        (rid, v), = drain(alone, 1).items()
+        assert got[rid][:2] == v[:2]

@@ -88,3 +88,43 @@ def test_oracle_sides_separately():
     neg = O.sieve_window(c["m"], c["b_min"], c["b_max"], c["p_max_pos"], c["p_max_neg"], mode=-1)
     both = O.sieve_window(c["m"], c["b_min"], c["b_max"], c["p_max_pos"], c["p_max_neg"], mode=0)
     assert np.array_equal(both, pos | neg)
+
+
+# ---------------- factor emission (sieve/GPU semantics) ----------------
+
+def brute_roots(p, m, plus):
+    """All b in [0, p) with p | N+(b) (plus) or N-(b), by direct evaluation."""
+    b = np.arange(p, dtype=np.int64)
+    y = b.copy()
+    for _ in range(m - 1):
+        y = (y * y) % p
+    v = (y * y - y + (1 if plus else -1)) % p
+    return set(int(x) for x in b[v == 0])
+
+
+@pytest.mark.parametrize("mode", [1, -1])
+def test_oracle_factors_match_brute_force(mode):
+    m = 8
+    p_min, p_max = (100000, 400000) if mode > 0 else (100000, 104000)
+    got = O.sieve_factors(m, p_min, p_max, mode)
+    by_p = {}
+    for p, b in got:
+        by_p.setdefault(int(p), set()).add(int(b))
+    primes = [int(p) for p in primes_upto(p_max) if p >= p_min and
+              ((mode > 0 and (p - 1) % (3 << m) == 0) or (mode < 0 and p % 10 in (1, 9)))]
+    assert primes
+    for p in primes:
+        assert by_p.get(p, set()) == brute_roots(p, m, mode > 0), p
+    assert set(by_p) <= set(primes)
+
+
+def test_oracle_factors_bmax_and_sizes():
+    """Large p: every emitted b is at most bMax and divides; + side counts."""
+    m = 12
+    got = O.sieve_factors(m, 10 ** 15, 10 ** 15 + 10 ** 10, 1)
+    assert len(got) > 0
+    for p, b in got[:200]:
+        p, b = int(p), int(b)
+        assert b <= 5 * 10 ** 9
+        x = pow(b, 1 << (m - 1), p)
+        assert (x * x - x + 1) % p == 0
