@@ -12,6 +12,7 @@
 #pragma once
 
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 #include "ct_arith.h"
@@ -33,27 +34,38 @@ inline tw_entry make_entry(uint32_t w_mont, int P)
 	return e;
 }
 
-// Returns 3 * 2 * n entries.
-inline std::vector<tw_entry> build_tables(uint32_t n)
+// Tables of prime P into tw (2 * n entries at tw[P * 2 * n]).
+inline void build_tables_prime(uint32_t n, int P, tw_entry *tw)
 {
-	std::vector<tw_entry> tw(6 * (size_t)n);
-	for (int P = 0; P < kPrimes; ++P)
 	{
 		const uint32_t p = kP[P], g = kG[P];
 		tw_entry *fw = &tw[(size_t)(P * 2 + 0) * n];
 		tw_entry *iw = &tw[(size_t)(P * 2 + 1) * n];
 		uint32_t r = 0;
-		for (uint32_t s = 1; s < n; s *= 2)
+		std::vector<uint32_t> pw;
+		// bitrev over log2(n/2) bits; for j < s, bitrev_s(j) = rev[j] >> (log2(n/2) - log2 s)
+		const uint32_t half = n / 2;
+		std::vector<uint32_t> rev(half);
+		rev[0] = 0;
+		for (uint32_t i = 1; i < half; ++i) rev[i] = (rev[i >> 1] >> 1) | ((i & 1) ? half >> 1 : 0);
+		int lhalf = 0;
+		while ((1u << lhalf) < half) ++lhalf;
+		int ls = 0;
+		for (uint32_t s = 1; s < n; s *= 2, ++ls)
 		{
+			// r_s has order 6s: all exponents needed are below 6s, and the
+			// inverse of r_s^e is r_s^(6s - e)
 			const uint32_t r_s = powmod_plain(g, (p - 1) / (6 * (uint64_t)s), p);
 			if (s == 1) r = r_s;
+			pw.resize(6 * (size_t)s);                 // Montgomery forms of r_s^t
+			const uint32_t rm = to_mont(r_s, p);
+			pw[0] = to_mont(1, p);
+			for (size_t t = 1; t < pw.size(); ++t) pw[t] = mont_mul(pw[t - 1], rm, p, kQ[P]);
 			for (uint32_t j = 0; j < s; ++j)
 			{
-				const uint32_t e = 3 * bitrev_n(j, s) + 1 + (2 * j) / s;
-				const uint32_t w = powmod_plain(r_s, e, p);
-				const uint32_t wi = powmod_plain(w, p - 2, p);
-				fw[s + j] = make_entry(to_mont(w, p), P);
-				iw[s + j] = make_entry(to_mont(wi, p), P);
+				const uint32_t e = 3 * (rev[j] >> (lhalf - ls)) + 1 + (2 * j) / s;
+				fw[s + j] = make_entry(pw[e], P);
+				iw[s + j] = make_entry(pw[(6 * s - e) % (6 * s)], P);
 			}
 		}
 		// merge constants: inverse of the level-1 split scaled by the
@@ -71,6 +83,20 @@ inline std::vector<tw_entry> build_tables(uint32_t n)
 		iw[1] = make_entry(to_mont(K, p), P);   // stored in Montgomery form
 		fw[0] = make_entry(0, P);
 	}
+}
+
+// Returns 3 * 2 * n entries; one host thread per prime for large n.
+inline std::vector<tw_entry> build_tables(uint32_t n)
+{
+	std::vector<tw_entry> tw(6 * (size_t)n);
+	if (n < (1u << 16))
+	{
+		for (int P = 0; P < kPrimes; ++P) build_tables_prime(n, P, tw.data());
+		return tw;
+	}
+	std::vector<std::thread> th;
+	for (int P = 0; P < kPrimes; ++P) th.emplace_back(build_tables_prime, n, P, tw.data());
+	for (auto &t : th) t.join();
 	return tw;
 }
 
