@@ -30,7 +30,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_run", "ct_prp_check", "ct_engine_timer_start", "ct_engine_timer_stop", "ct_engine_info",
            "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes",
            "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
-           "ct_server_stats", "ct_sieve_factors"]
+           "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli"]
 
 
 class CtError(RuntimeError):
@@ -86,6 +86,7 @@ def lib():
                                c.POINTER(u32)],
             "ct_server_stats": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u64), c.POINTER(u64)],
             "ct_engine_primes": [vp, c.POINTER(i32)],
+            "ct_engine_moduli": [u32p],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -257,6 +258,13 @@ class Server:
         q = ctypes.c_uint32(); a = ctypes.c_uint32(); u = ctypes.c_uint64(); s = ctypes.c_uint64()
         _check(lib().ct_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(u), ctypes.byref(s)))
         return {"queued": q.value, "active": a.value, "hbm_used": u.value, "squarings": s.value}
+
+
+def moduli():
+    """The engine's three NTT primes, largest first (host call, no GPU)."""
+    p = np.zeros(3, np.uint32)
+    _check(lib().ct_engine_moduli(p))
+    return [int(x) for x in p]
 
 
 def res64_string(r):

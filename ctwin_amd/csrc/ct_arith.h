@@ -1,13 +1,16 @@
 // Exact modular and multi-word integer arithmetic shared by the gfx950 kernels
 // and the host code of the squaring engine.
 //
-// Residues live modulo three primes p < 2^31 with 3 * 2^23 | p - 1, so that the
-// negacyclic-like transform modulo X^n - X^(n/2) + 1 exists for n <= 2^23 and the
-// "min trick" reduces a sum or difference in three VALU instructions
-// (a + b < 2^32 because 2p < 2^32).  Products use Montgomery reduction with
-// R = 2^32 (P. L. Montgomery, Math. Comp. 44 (1985) 519-521); a multiply by a
-// constant w uses the precomputed wq = w * p^-1 mod 2^32 so that it costs
-// mul_lo + two mul_hi (all three are full-rate VALU on gfx950).
+// Residues live modulo three primes p < 2^30 with 3 * 2^22 | p - 1, so that the
+// transform modulo X^n - X^(n/2) + 1 exists for n <= 2^22 and the butterflies
+// can work lazily on representatives in [0, 4p) (4p < 2^32; D. Harvey, "Faster
+// arithmetic for number-theoretic transforms", J. Symbolic Comput. 60 (2014)
+// 113-119).  A product by a table constant w uses Shoup's precomputed quotient
+// w' = floor(w 2^32 / p) (V. Shoup, NTL): q = mulhi(a, w'), a w - q p lies in
+// [0, 2p) for every 32-bit a -- three or four VALU instructions.  Products of
+// two variables (the pointwise square) use Montgomery reduction with R = 2^32
+// (P. L. Montgomery, Math. Comp. 44 (1985) 519-521); the CRT keeps the
+// reduced [0, p) forms.
 #pragma once
 
 #include <cstdint>
@@ -23,9 +26,9 @@ namespace ct {
 // ---- prime set ------------------------------------------------------------
 
 constexpr int kPrimes = 3;
-constexpr uint32_t kP[kPrimes] = {2113929217u, 2088763393u, 2013265921u};   // 63*2^25+1, 249*2^23+1, 15*2^27+1
-constexpr uint32_t kG[kPrimes] = {5u, 5u, 31u};                              // primitive roots
-constexpr int kMaxLog2N = 23;                                                 // 3*2^23 | p-1 for all three
+constexpr uint32_t kP[kPrimes] = {943718401u, 918552577u, 880803841u};   // (75, 73, 70) * 3 * 2^22 + 1
+constexpr uint32_t kG[kPrimes] = {7u, 5u, 26u};                           // primitive roots
+constexpr int kMaxLog2N = 22;                                              // 3*2^22 | p-1 for all three
 
 constexpr uint32_t inv32(uint32_t p)   // p^-1 mod 2^32 by Newton iteration
 {
@@ -36,9 +39,10 @@ constexpr uint32_t inv32(uint32_t p)   // p^-1 mod 2^32 by Newton iteration
 
 constexpr uint32_t kQ[kPrimes] = {inv32(kP[0]), inv32(kP[1]), inv32(kP[2])};
 static_assert(kQ[0] * kP[0] == 1u && kQ[1] * kP[1] == 1u && kQ[2] * kP[2] == 1u, "Montgomery inverse");
-static_assert(kP[0] < 0x80000000u && kP[1] < 0x80000000u && kP[2] < 0x80000000u, "min trick needs p < 2^31");
+static_assert(kP[0] < 0x40000000u && kP[1] < 0x40000000u && kP[2] < 0x40000000u, "lazy [0, 4p) needs p < 2^30");
+static_assert(kP[0] > kP[1] && kP[1] > kP[2], "the CRT assumes p1 > p2 > p3");
 
-// ---- mod p ----------------------------------------------------------------
+// ---- mod p, reduced forms [0, p) (CRT) ----------------------------------------
 
 CT_HD uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
@@ -68,8 +72,33 @@ CT_HD uint32_t mont_mulc(uint32_t a, uint32_t w, uint32_t wq, uint32_t p)
 	return umin32(r, r + p);
 }
 
-// Signed 32-bit digit -> residue.
-CT_HD uint32_t from_signed(int32_t d, uint32_t p) { return (d < 0) ? (uint32_t)((int64_t)d + p) : (uint32_t)d; }
+// ---- lazy forms (transform) -----------------------------------------------------
+
+// [0, 4p) -> [0, 2p) and [0, 2p) -> [0, p): one subtract and one unsigned min
+// (the subtraction wraps to a large value when no correction is due).
+CT_HD uint32_t red2p(uint32_t a, uint32_t p) { return umin32(a, a - 2 * p); }
+CT_HD uint32_t red1p(uint32_t a, uint32_t p) { return umin32(a, a - p); }
+
+// Shoup product: a w mod p in [0, 2p) for any 32-bit a, w in [0, p), wp = floor(w 2^32 / p).
+CT_HD uint32_t shoup(uint32_t a, uint32_t w, uint32_t wp, uint32_t p) { return a * w - mulhi32(a, wp) * p; }
+
+// Negated Shoup product: with wn = 2^32 - w, returns 2^32 - t (mod 2^32) where
+// t = shoup(a, w, wp, p); so x - result = x + t and x + result + 2p = x - t + 2p.
+// (a wn + q p is one mul_lo and one mad_u64_u32 on gfx950.)
+CT_HD uint32_t shoup_neg(uint32_t a, uint32_t wn, uint32_t wp, uint32_t p) { return a * wn + mulhi32(a, wp) * p; }
+
+// Montgomery product of a, b in [0, 2p): a b 2^-32 mod p in (0, 2p).
+CT_HD uint32_t mont_lazy(uint32_t a, uint32_t b, uint32_t p, uint32_t q)
+{
+	const uint64_t t = (uint64_t)a * b;                 // < 4 p^2
+	const uint32_t m = (uint32_t)t * q;
+	return (uint32_t)(t >> 32) - mulhi32(m, p) + p;     // (t - m p) / 2^32 in (-p, p)
+}
+
+// Signed 32-bit digit -> representative in [0, 3p): d + 3p for d < 0 (d >= -2^31
+// and 3p > 2^31 keep it non-negative; 3p < 2^32), d itself otherwise (d < 2^31 < 3p).
+static_assert(3ull * kP[2] > 0x80000000ull && 3ull * kP[0] < 0x100000000ull, "digit range");
+CT_HD uint32_t from_signed(int32_t d, uint32_t p) { return (d < 0) ? (uint32_t)d + 3 * p : (uint32_t)d; }
 
 // ---- host-side (or constexpr) plain modular helpers -------------------------
 

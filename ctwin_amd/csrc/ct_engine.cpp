@@ -45,7 +45,7 @@ int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize
 {
 	if (!out) return fail(CT_ERR_ARG, "out is null");
 	*out = nullptr;
-	if (log2_n < 5 || log2_n > (uint32_t)kMaxLog2N) return fail(CT_ERR_UNSUPPORTED, "log2_n must be in [5, 23]");
+	if (log2_n < 5 || log2_n > (uint32_t)kMaxLog2N) return fail(CT_ERR_UNSUPPORTED, "log2_n must be in [5, 22]");
 	if (lanes == 0) return fail(CT_ERR_ARG, "lanes must be > 0");
 	if (csize != 0 && csize != 16) return fail(CT_ERR_UNSUPPORTED, "csize must be 0 or 16");
 	if (kind == CT_KIND_AUTO) kind = (log2_n <= 14) ? CT_KIND_RESIDENT : CT_KIND_PLAIN;
@@ -105,6 +105,13 @@ int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind)
 	if (n) *n = e->n;
 	if (lanes) *lanes = e->lanes;
 	if (kind) *kind = e->kind;
+	return CT_OK;
+}
+
+int ct_engine_moduli(uint32_t *p)
+{
+	if (!p) return fail(CT_ERR_ARG, "null pointer");
+	for (int i = 0; i < kPrimes; ++i) p[i] = kP[i];
 	return CT_OK;
 }
 

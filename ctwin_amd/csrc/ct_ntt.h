@@ -92,6 +92,13 @@ __device__ __forceinline__ void xchg(V (&u)[E], V *lds, uint32_t tt)
 }
 
 // ---- radix-16 DIF pass over the instance's levels 4L+1 .. 4L+4 -------------
+//
+// Value ranges (lazy butterflies, p < 2^30):
+//   forward (x, y) -> (x + w y, x - w y): x, y in [0, 4p) -> [0, 4p)
+//     (x is brought to [0, 2p), t = w y in [0, 2p) by Shoup);
+//   inverse (x, y) -> (x + y, (x - y) w): x, y in [0, 2p) -> [0, 2p);
+//   the level-1 (Phi_6) forward split takes inputs in [0, 3p) (digits), the
+//   level-1 merge returns reduced values in [0, p).
 
 template <int LNL, int P, int L, bool PHI6>
 __device__ __forceinline__ void fwd_pass(uint32_t (&u)[E], const tw_entry *__restrict__ twf, uint32_t base, uint32_t tt)
@@ -112,16 +119,19 @@ __device__ __forceinline__ void fwd_pass(uint32_t (&u)[E], const tw_entry *__res
 		{
 			if ((v & h) != 0) continue;
 			const tw_entry wv = w[v >> (4 - q)];
-			const uint32_t x = u[v], y = mont_mulc(u[v + h], wv.w, wv.wq, p);
+			const uint32_t x = red2p(u[v], p);
+			const uint32_t s = shoup_neg(u[v + h], wv.w, wv.wp, p);     // 2^32 - w y'
 			if (PHI6 && L == 0 && q == 0)
-			{   // level 1: mod X^(n/2) - r and X^(n/2) - r', r' = 1 - r
-				u[v] = add_mod(x, y, p);
-				u[v + h] = add_mod(u[v + h], sub_mod(x, y, p), p);
+			{   // level 1: mod X^(n/2) - r and X^(n/2) - r', r' = 1 - r:
+				// (x + r y, y + x - r y)
+				const uint32_t yx = red2p(red2p(u[v + h], p) + x, p);
+				u[v] = x - s;
+				u[v + h] = yx + s + 2 * p;
 			}
 			else
 			{
-				u[v] = add_mod(x, y, p);
-				u[v + h] = sub_mod(x, y, p);
+				u[v] = x - s;
+				u[v + h] = x + s + 2 * p;
 			}
 		}
 	}
@@ -145,9 +155,9 @@ __device__ __forceinline__ void inv_pass(uint32_t (&u)[E], const tw_entry *__res
 			for (int v = 0; v < h; ++v)
 			{
 				const uint32_t x = u[v], y = u[v + h];
-				const uint32_t vh = mont_mulc(add_mod(x, y, p), r.w, r.wq, p);
-				u[v + h] = mont_mulc(sub_mod(y, x, p), K.w, K.wq, p);
-				u[v] = mont_mulc(sub_mod(x, vh, p), K.w, K.wq, p);
+				const uint32_t vh = shoup(x + y, r.w, r.wp, p);
+				u[v + h] = red1p(shoup(y + 2 * p - x, K.w, K.wp, p), p);
+				u[v] = red1p(shoup(x + 2 * p - vh, K.w, K.wp, p), p);
 			}
 			continue;
 		}
@@ -160,8 +170,8 @@ __device__ __forceinline__ void inv_pass(uint32_t (&u)[E], const tw_entry *__res
 			if ((v & h) != 0) continue;
 			const tw_entry wv = w[v >> (4 - q)];
 			const uint32_t x = u[v], y = u[v + h];
-			u[v] = add_mod(x, y, p);
-			u[v + h] = mont_mulc(sub_mod(x, y, p), wv.w, wv.wq, p);
+			u[v] = red2p(x + y, p);
+			u[v + h] = shoup(x + 2 * p - y, wv.w, wv.wp, p);
 		}
 	}
 }
@@ -188,13 +198,13 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 			if ((z & h) != 0) continue;
 			const tw_entry w = tq[z >> (f + 1 - q)];
 			{
-				const uint32_t x = u[z], y = mont_mulc(u[z + h], w.w, w.wq, p);
-				u[z] = add_mod(x, y, p); u[z + h] = sub_mod(x, y, p);
+				const uint32_t x = red2p(u[z], p), sn = shoup_neg(u[z + h], w.w, w.wp, p);
+				u[z] = x - sn; u[z + h] = x + sn + 2 * p;
 			}
 			if (MUL)
 			{
-				const uint32_t x = vv[z], y = mont_mulc(vv[z + h], w.w, w.wq, p);
-				vv[z] = add_mod(x, y, p); vv[z + h] = sub_mod(x, y, p);
+				const uint32_t x = red2p(vv[z], p), sn = shoup_neg(vv[z + h], w.w, w.wp, p);
+				vv[z] = x - sn; vv[z + h] = x + sn + 2 * p;
 			}
 		}
 	}
@@ -204,19 +214,20 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 	for (int z = 0; z < E; z += 2)
 	{
 		const tw_entry w = tl[z >> 1];
-		const uint32_t u0 = u[z], u1 = u[z + 1];
-		const uint32_t u1w = mont_mulc(u1, w.w, w.wq, p);
+		const uint32_t u0 = red2p(u[z], p), u1 = red2p(u[z + 1], p);
+		const uint32_t u1w = 0u - shoup_neg(u[z + 1], w.w, w.wp, p);             // w u1 in [0, 2p)
 		if (!MUL)
 		{
-			u[z + 1] = mont_mul(add_mod(u0, u0, p), u1, p, pq);
-			u[z] = add_mod(mont_mul(u0, u0, p, pq), mont_mul(u1w, u1w, p, pq), p);
+			const uint32_t c = mont_lazy(u0, u1, p, pq);
+			u[z + 1] = red2p(c + c, p);
+			u[z] = red2p(mont_lazy(u0, u0, p, pq) + mont_lazy(u1w, u1w, p, pq), p);
 		}
 		else
 		{
-			const uint32_t v0 = vv[z], v1 = vv[z + 1];
-			const uint32_t v1w = mont_mulc(v1, w.w, w.wq, p);
-			u[z + 1] = add_mod(mont_mul(u0, v1, p, pq), mont_mul(u1, v0, p, pq), p);
-			u[z] = add_mod(mont_mul(u0, v0, p, pq), mont_mul(u1w, v1w, p, pq), p);
+			const uint32_t v0 = red2p(vv[z], p), v1 = red2p(vv[z + 1], p);
+			const uint32_t v1w = 0u - shoup_neg(vv[z + 1], w.w, w.wp, p);
+			u[z + 1] = red2p(mont_lazy(u0, v1, p, pq) + mont_lazy(u1, v0, p, pq), p);
+			u[z] = red2p(mont_lazy(u0, v0, p, pq) + mont_lazy(u1w, v1w, p, pq), p);
 		}
 	}
 #pragma unroll
@@ -231,8 +242,8 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 			if ((z & h) != 0) continue;
 			const tw_entry w = tq[z >> (f + 1 - q)];
 			const uint32_t x = u[z], y = u[z + h];
-			u[z] = add_mod(x, y, p);
-			u[z + h] = mont_mulc(sub_mod(x, y, p), w.w, w.wq, p);
+			u[z] = red2p(x + y, p);
+			u[z + h] = shoup(x + 2 * p - y, w.w, w.wp, p);
 		}
 	}
 }

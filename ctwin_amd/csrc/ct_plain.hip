@@ -32,7 +32,7 @@ struct plain_args
 	const int32_t *xb;          // second operand digits (product only)
 	uint32_t *res;              // [3][lanes][n] residue planes
 	uint32_t *res2;             // second operand planes (product only)
-	int64_t *carry;             // [lanes][n/16] chunk carries
+	int64_t *carry;             // [lanes][n/16] chunk carries, chunk g at cslot(g)
 	const int64_t *carry_in;    // pending carries to apply on load (or null)
 	const tw_entry *tw;
 	const divisor *dv;
@@ -46,23 +46,33 @@ struct plain_args
 	garner_consts gc;
 };
 
-template <int RL>
+// Column-pass geometry: W adjacent columns per workgroup (64-byte row
+// segments for W = 16, 128-byte ones for W = 32 at large n).
+template <int LN, int RL>
 struct colgeo
 {
 	static constexpr uint32_t R = 1u << RL;
 	static constexpr uint32_t Tc = R / 16;                 // threads per column
-	static constexpr uint32_t W = (RL == 4) ? 64 : 16;      // columns per workgroup
+	static constexpr uint32_t W = (RL == 4) ? 64 : ((LN >= 19) ? 32 : 16);   // columns per workgroup
 	static constexpr uint32_t threads = W * Tc;            // = chunks per workgroup (R W / 16)
 	static constexpr uint32_t RP = R + (R >> 5);           // padded column stride in LDS
 	static constexpr int passes = RL / 4;
 };
 
 // column forward passes 0..passes-1 (transposes between them only)
+// Slot of chunk g (16 digits; row g / cpr, chunk g % cpr of the row, cpr =
+// chunks per row) in a lane's carry array: chunk-column major, so that the
+// column passes, whose threads hold one chunk per row, touch consecutive slots.
+__host__ __device__ __forceinline__ uint32_t cslot(uint32_t g, uint32_t cpr, uint32_t rows)
+{
+	return (g % cpr) * rows + g / cpr;
+}
+
 template <int RL, int P, int L>
 __device__ __forceinline__ void col_fwd(uint32_t (&u)[E], uint32_t *X, const tw_entry *twf, uint32_t tt)
 {
 	fwd_pass<RL, P, L, true>(u, twf, 1u, tt);
-	if constexpr (L + 1 < colgeo<RL>::passes)
+	if constexpr (L + 1 < RL / 4)
 	{
 		xchg<RL, L, L + 1>(u, X, tt);
 		col_fwd<RL, P, L + 1>(u, X, twf, tt);
@@ -84,7 +94,7 @@ template <int LN, int RL, int NP>
 __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, uint32_t *res_lane, const tw_entry *tw,
                                                uint32_t c0, uint32_t cw, uint32_t tt, size_t plane)
 {
-	using G = colgeo<RL>;
+	using G = colgeo<LN, RL>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL;
 	constexpr int La = G::passes - 1;
 #pragma unroll
@@ -108,9 +118,9 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 
 // A: chunk load (+ pending carry), column levels, residue planes
 template <int LN, int RL, int NP>
-__global__ __launch_bounds__(colgeo<RL>::threads) void k_colA(plain_args A, int operand)
+__global__ __launch_bounds__((colgeo<LN, RL>::threads)) void k_colA(plain_args A, int operand)
 {
-	using G = colgeo<RL>;
+	using G = colgeo<LN, RL>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
 	__shared__ int32_t D[G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
@@ -131,9 +141,9 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colA(plain_args A, int 
 		{
 			const uint32_t g = k / 16;
 			const int64_t *f = A.carry_in + (size_t)lane * nc;
-			int64_t c = f[(g + nc - 1) % nc];
+			int64_t c = f[cslot((g + nc - 1) % nc, C / 16, G::R)];
 			if (g == 0) c = -c;
-			else if (g == nc / 2) c += f[nc - 1];
+			else if (g == nc / 2) c += f[cslot(nc - 1, C / 16, G::R)];
 			carry_in(dig, c, A.dv[lane]);
 		}
 #pragma unroll
@@ -190,9 +200,9 @@ __global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
 
 // C: inverse column levels, CRT, normalize1
 template <int LN, int RL, int NP>
-__global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int no_dup)
+__global__ __launch_bounds__((colgeo<LN, RL>::threads)) void k_colC(plain_args A, int no_dup)
 {
-	using G = colgeo<RL>;
+	using G = colgeo<LN, RL>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
 	constexpr int La = G::passes - 1;
 	__shared__ uint32_t S[3][G::W * G::RP];
@@ -258,20 +268,20 @@ __global__ __launch_bounds__(colgeo<RL>::threads) void k_colC(plain_args A, int 
 	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * n + k);
 #pragma unroll
 	for (int v = 0; v < 4; ++v) dst[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
-	A.carry[(size_t)lane * nc + k / 16] = f;
+	A.carry[(size_t)lane * nc + cslot(k / 16, C / 16, G::R)] = f;
 }
 
 // normalize2 alone: materialises the pending carries into the digits
-__global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, uint32_t n, uint32_t lanes)
+__global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, uint32_t n, uint32_t lanes, uint32_t rows)
 {
-	const uint32_t nc = n / 16;
+	const uint32_t nc = n / 16, cpr = nc / rows;
 	const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (id >= (size_t)nc * lanes) return;
 	const uint32_t lane = (uint32_t)(id / nc), g = (uint32_t)(id % nc);
 	const int64_t *f = carry + (size_t)lane * nc;
-	int64_t c = f[(g + nc - 1) % nc];
+	int64_t c = f[cslot((g + nc - 1) % nc, cpr, rows)];
 	if (g == 0) c = -c;
-	else if (g == nc / 2) c += f[nc - 1];
+	else if (g == nc / 2) c += f[cslot(nc - 1, cpr, rows)];
 	int4 *p = reinterpret_cast<int4 *>(x + (size_t)lane * n + 16 * (size_t)g);
 	int32_t dig[E];
 #pragma unroll
@@ -289,7 +299,7 @@ static int rows_log2(int ln) { return (ln <= 12) ? 4 : 8; }
 template <int LN, int RL, int NP>
 static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	using G = colgeo<RL>;
+	using G = colgeo<LN, RL>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
 	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, A, 0);
@@ -301,7 +311,7 @@ static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
 template <int LN, int RL, int NP>
 static hipError_t mul_ln(const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
 {
-	using G = colgeo<RL>;
+	using G = colgeo<LN, RL>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
 	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, In, 0);
@@ -383,7 +393,7 @@ int plain_materialize(ct_engine *e)
 	if (!e->pending) return CT_OK;
 	const size_t total = (size_t)e->lanes * (e->n / 16);
 	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,
-	                   e->reg(0), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes);
+	                   e->reg(0), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes, 1u << e->rows_log2);
 	HIPCHK(hipGetLastError());
 	e->pending = false;
 	return CT_OK;
@@ -419,7 +429,7 @@ int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
 	// materialise dst at once (only register 0 may carry pending carries)
 	const size_t total = (size_t)e->lanes * (e->n / 16);
 	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,
-	                   e->reg(dst), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes);
+	                   e->reg(dst), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes, 1u << e->rows_log2);
 	HIPCHK(hipGetLastError());
 	return CT_OK;
 }

@@ -4,11 +4,13 @@
 // X^n - X^(n/2) + 1 = (X^(n/2) - r)(X^(n/2) - r') with r a primitive 6th root of
 // unity (r + r' = 1); below it, block j of level s (s blocks) is reduced by its
 // root w[s + j] = r_s^(3 bitrev_s(j) + 1 + floor(2j/s)), r_s = g^((p-1)/(6s)).
-// The inverse table holds w[s + j]^-1.  Entry 0/1 of the inverse table carry
+// The inverse table holds w[s + j]^-1.  Entries 0/1 of the inverse table carry
 // the level-1 merge constants.
 //
-// Table layout (uint2 entries = (w * 2^32 mod p, that value * p^-1 mod 2^32)):
-//   tw[((P * 2) + dir) * n + idx], P = prime, dir 0 = forward, 1 = inverse.
+// Table layout: tw[((P * 2) + dir) * n + idx], P = prime, dir 0 = forward,
+// 1 = inverse; each entry is a Shoup pair (a, w') with w' = floor(w 2^32 / p)
+// and a = 2^32 - w in the forward table (negated product, see shoup_neg), a = w
+// in the inverse table.
 #pragma once
 
 #include <cstdint>
@@ -19,70 +21,68 @@
 
 namespace ct {
 
-struct tw_entry { uint32_t w, wq; };
+struct tw_entry { uint32_t w, wp; };
 
-inline uint32_t bitrev_n(uint32_t i, uint32_t n)
+// floor(w 2^32 / p) for w < p < 2^30, from a double estimate corrected exactly.
+inline uint32_t shoup_quot(uint32_t w, uint32_t p)
 {
-	uint32_t r = 0;
-	for (uint32_t k = n, j = i; k > 1; k /= 2, j /= 2) r = (2 * r) | (j % 2);
-	return r;
+	const unsigned __int128 num = (unsigned __int128)w << 32;
+	uint64_t q = (uint64_t)((double)w * (4294967296.0 / (double)p));
+	while ((unsigned __int128)q * p > num) --q;
+	while ((unsigned __int128)(q + 1) * p <= num) ++q;
+	return (uint32_t)q;
 }
 
-inline tw_entry make_entry(uint32_t w_mont, int P)
-{
-	tw_entry e; e.w = w_mont; e.wq = w_mont * kQ[P];
-	return e;
-}
+inline tw_entry fwd_entry(uint32_t w, uint32_t p) { tw_entry e; e.w = 0u - w; e.wp = shoup_quot(w, p); return e; }
+inline tw_entry inv_entry(uint32_t w, uint32_t p) { tw_entry e; e.w = w; e.wp = shoup_quot(w, p); return e; }
 
 // Tables of prime P into tw (2 * n entries at tw[P * 2 * n]).
 inline void build_tables_prime(uint32_t n, int P, tw_entry *tw)
 {
+	const uint32_t p = kP[P], g = kG[P];
+	tw_entry *fw = &tw[(size_t)(P * 2 + 0) * n];
+	tw_entry *iw = &tw[(size_t)(P * 2 + 1) * n];
+	uint32_t r = 0;
+	std::vector<uint32_t> pw;
+	// bitrev over log2(n/2) bits; for j < s, bitrev_s(j) = rev[j] >> (log2(n/2) - log2 s)
+	const uint32_t half = n / 2;
+	std::vector<uint32_t> rev(half);
+	rev[0] = 0;
+	for (uint32_t i = 1; i < half; ++i) rev[i] = (rev[i >> 1] >> 1) | ((i & 1) ? half >> 1 : 0);
+	int lhalf = 0;
+	while ((1u << lhalf) < half) ++lhalf;
+	int ls = 0;
+	for (uint32_t s = 1; s < n; s *= 2, ++ls)
 	{
-		const uint32_t p = kP[P], g = kG[P];
-		tw_entry *fw = &tw[(size_t)(P * 2 + 0) * n];
-		tw_entry *iw = &tw[(size_t)(P * 2 + 1) * n];
-		uint32_t r = 0;
-		std::vector<uint32_t> pw;
-		// bitrev over log2(n/2) bits; for j < s, bitrev_s(j) = rev[j] >> (log2(n/2) - log2 s)
-		const uint32_t half = n / 2;
-		std::vector<uint32_t> rev(half);
-		rev[0] = 0;
-		for (uint32_t i = 1; i < half; ++i) rev[i] = (rev[i >> 1] >> 1) | ((i & 1) ? half >> 1 : 0);
-		int lhalf = 0;
-		while ((1u << lhalf) < half) ++lhalf;
-		int ls = 0;
-		for (uint32_t s = 1; s < n; s *= 2, ++ls)
+		// r_s has order 6s: all exponents needed are below 6s, and the
+		// inverse of r_s^e is r_s^(6s - e)
+		const uint32_t r_s = powmod_plain(g, (p - 1) / (6 * (uint64_t)s), p);
+		if (s == 1) r = r_s;
+		pw.resize(6 * (size_t)s);                 // r_s^t, t < 6s
+		const uint32_t rp = shoup_quot(r_s, p);
+		pw[0] = 1;
+		for (size_t t = 1; t < pw.size(); ++t) pw[t] = red1p(shoup(pw[t - 1], r_s, rp, p), p);
+		for (uint32_t j = 0; j < s; ++j)
 		{
-			// r_s has order 6s: all exponents needed are below 6s, and the
-			// inverse of r_s^e is r_s^(6s - e)
-			const uint32_t r_s = powmod_plain(g, (p - 1) / (6 * (uint64_t)s), p);
-			if (s == 1) r = r_s;
-			pw.resize(6 * (size_t)s);                 // Montgomery forms of r_s^t
-			const uint32_t rm = to_mont(r_s, p);
-			pw[0] = to_mont(1, p);
-			for (size_t t = 1; t < pw.size(); ++t) pw[t] = mont_mul(pw[t - 1], rm, p, kQ[P]);
-			for (uint32_t j = 0; j < s; ++j)
-			{
-				const uint32_t e = 3 * (rev[j] >> (lhalf - ls)) + 1 + (2 * j) / s;
-				fw[s + j] = make_entry(pw[e], P);
-				iw[s + j] = make_entry(pw[(6 * s - e) % (6 * s)], P);
-			}
+			const uint32_t e = 3 * (rev[j] >> (lhalf - ls)) + 1 + (2 * j) / s;
+			fw[s + j] = fwd_entry(pw[e], p);
+			iw[s + j] = inv_entry(pw[(6 * s - e) % (6 * s)], p);
 		}
-		// merge constants: inverse of the level-1 split scaled by the
-		// accumulated 2^(log2 n - 2) of the unnormalised inverse butterflies and
-		// by the Montgomery factor 2^-32 left by the pointwise products:
-		//   u1 = (a - b) / (2r - 1), u0 = a - r u1.  We apply
-		//   x[i + n/2] = (b - a) * K, x[i] = (a - (a + b) r) * K,
-		//   K = 2 (1 - 2r)^-1 * (2 / n) * 2^32, stored as K * 2^32 mod p.
-		const uint32_t one_m_2r = (uint32_t)(((uint64_t)p + 1 - (2 * (uint64_t)r) % p) % p);
-		const uint32_t inv = powmod_plain(one_m_2r, p - 2, p);
-		const uint32_t two_over_n = powmod_plain(n / 2, p - 2, p);
-		uint32_t K = mulmod_plain(mulmod_plain(2, inv, p), two_over_n, p);
-		K = to_mont(K, p);                 // * 2^32 (cancels the products' 2^-32)
-		iw[0] = make_entry(to_mont(r, p), P);
-		iw[1] = make_entry(to_mont(K, p), P);   // stored in Montgomery form
-		fw[0] = make_entry(0, P);
 	}
+	// merge constants: inverse of the level-1 split scaled by the
+	// accumulated 2^(log2 n - 2) of the unnormalised inverse butterflies and
+	// by the Montgomery factor 2^-32 left by the pointwise products:
+	//   u1 = (a - b) / (2r - 1), u0 = a - r u1.  We apply
+	//   x[i + n/2] = (b - a) * K, x[i] = (a - (a + b) r) * K,
+	//   K = 2 (1 - 2r)^-1 * (2 / n) * 2^32 mod p.
+	const uint32_t one_m_2r = (uint32_t)(((uint64_t)p + 1 - (2 * (uint64_t)r) % p) % p);
+	const uint32_t inv = powmod_plain(one_m_2r, p - 2, p);
+	const uint32_t two_over_n = powmod_plain(n / 2, p - 2, p);
+	uint32_t K = mulmod_plain(mulmod_plain(2, inv, p), two_over_n, p);
+	K = to_mont(K, p);                 // * 2^32 (cancels the products' 2^-32)
+	iw[0] = inv_entry(r, p);
+	iw[1] = inv_entry(K, p);
+	fw[0].w = 0; fw[0].wp = 0;
 }
 
 // Returns 3 * 2 * n entries; one host thread per prime for large n.

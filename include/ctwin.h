@@ -39,7 +39,7 @@ typedef struct ct_engine ct_engine;
 
 /* Replaces engine(platform, d, verbose) + transform(size, engine, isBoinc,
  * csize) (cyclo/src/engine.h:40, cyclo/src/transform.h:380-433): device
- * ordinal, log2 n (5..23), number of candidates ("lanes"; the reference
+ * ordinal, log2 n (5..22; resident kernels to 14, multipass from 10), number of candidates ("lanes"; the reference
  * fixes VSIZE = 64, engine.h:19), carry chunk csize (0 or 16; the reference
  * autotunes 8..64, transform.h:383-419; the carried digits depend on it, the
  * residues do not), engine kind.  Allocates 3 digit registers per lane. */
@@ -115,6 +115,12 @@ int ct_engine_primes(const ct_engine *e, int *np);
 
 /* Engine facts: n, lanes, kind actually used. */
 int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind);
+
+/* The engine's NTT primes p[0] > p[1] > p[2] (host only, no device needed).
+ * The reference fixes its primes as #defines (transform.h:26-29); this build
+ * uses primes below 2^30 so that its butterflies can stay lazy.  Two primes
+ * serve when 3 n D^2 < p[0] p[1] / 2 (see ct_engine_set_primes). */
+int ct_engine_moduli(uint32_t *p);
 
 /* ---------------- serving loop ---------------- */
 

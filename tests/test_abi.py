@@ -41,3 +41,24 @@ def test_errors_without_device_are_status_codes():
     assert L.ct_engine_create(0, 10, 0, 16, 0, ctypes.byref(h)) == -1
     assert L.ct_engine_create(0, 10, 1, 8, 0, ctypes.byref(h)) == -4
     assert b"log2_n" in L.ct_last_error() or len(L.ct_last_error()) > 0
+
+
+def test_moduli_are_ntt_primes_below_2_30():
+    """The engine's primes: below 2^30 (lazy butterflies need 4p < 2^32),
+    descending (the CRT assumes p1 > p2 > p3), 3 * 2^22 | p - 1 (the transform
+    of length 2^22 modulo X^n - X^(n/2) + 1 needs a root of order 3n), prime,
+    and large enough that three of them hold 3 n b^2 for n = 2^22, b < 2^31."""
+    if not os.path.exists(ctwin_amd.LIB_PATH):
+        pytest.skip("libctwin_amd.so not built")
+    P = ctwin_amd.moduli()
+    assert P == sorted(P, reverse=True) and len(set(P)) == 3
+    for p in P:
+        assert p < 1 << 30 and (p - 1) % (3 << 22) == 0
+        assert all(pow(a, p - 1, p) == 1 for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37))
+        d, s = p - 1, 0
+        while d % 2 == 0:
+            d //= 2; s += 1
+        for a in (2, 3, 5, 7):
+            x = pow(a, d, p)
+            assert x in (1, p - 1) or any(pow(x, 2 ** k, p) == p - 1 for k in range(1, s))
+    assert 3 * (1 << 22) * (1 << 31) ** 2 < P[0] * P[1] * P[2] // 2

@@ -22,7 +22,7 @@ def oracle_engines(n, bs):
 
 def expected_primes(n, bs, np_mode):
     D = max(bs)
-    return 3 if np_mode == 3 or 3 * n * D * D >= (2113929217 * 2088763393) // 2 else 2
+    return 3 if np_mode == 3 or 3 * n * D * D >= (ct.moduli()[0] * ct.moduli()[1]) // 2 else 2
 
 
 @pytest.mark.parametrize("np_mode", [0, 3])
@@ -134,7 +134,7 @@ def test_prp_check_known_prp():
 # ---------------- plain (multipass) engine ----------------
 
 @pytest.mark.parametrize("np_mode", [0, 3])
-@pytest.mark.parametrize("ln", [10, 11, 12, 13, 14, 16])
+@pytest.mark.parametrize("ln", [10, 11, 12, 13, 14, 16, 19])
 def test_plain_square_dup_digit_exact(ln, np_mode):
     n = 1 << ln
     rng = random.Random(300 + ln)
