@@ -91,6 +91,99 @@ __device__ __forceinline__ void xchg(V (&u)[E], V *lds, uint32_t tt)
 	for (int q = 0; q < E; ++q) u[q] = rd[loff<LNL, TO>(q)];
 }
 
+// ---- twiddles of one pass, loaded one pass ahead of their use -------------
+//
+// A radix-16 pass over levels 4L+1..4L+4 of an instance needs 1 + 2 + 4 + 8
+// roots per thread: level q (0..3) uses tw[(sj << q) + k], k < 2^q, with
+// sj = (base << 4L) + j.  They are loaded into registers while the previous
+// pass computes, so the global-memory latency is not exposed between passes
+// (the passes are separated by LDS transposes and workgroup barriers).
+
+struct ptw                                // roots of one pass held in registers
+{
+	tw_entry w[15];                       // level q, root k at w[(1 << q) - 1 + k]
+	__device__ __forceinline__ tw_entry get(int q, int k) const { return w[(1 << q) - 1 + k]; }
+};
+struct ptw_lazy                           // the same roots read where they are used
+{
+	const tw_entry *tw; uint32_t sj;
+	__device__ __forceinline__ tw_entry get(int q, int k) const { return tw[(sj << q) + k]; }
+};
+struct mtw { tw_entry r, K; };           // level-1 merge constants (inverse table entries 0, 1)
+
+template <int LNL, int L>
+__device__ __forceinline__ ptw_lazy lazy_ptw(const tw_entry *tw, uint32_t base, uint32_t tt)
+{
+	constexpr uint32_t mm = (geo<LNL>::n >> (4 * L)) / 16;
+	ptw_lazy T; T.tw = tw; T.sj = (base << (4 * L)) + ((L == 0) ? 0u : tt / mm);
+	return T;
+}
+
+template <int LNL, int L>
+__device__ __forceinline__ void load_ptw(ptw &T, const tw_entry *__restrict__ tw, uint32_t base, uint32_t tt)
+{
+	constexpr uint32_t mm = (geo<LNL>::n >> (4 * L)) / 16;
+	const uint32_t j = (L == 0) ? 0u : tt / mm;
+	const uint32_t sj = (base << (4 * L)) + j;
+#pragma unroll
+	for (int q = 0; q < 4; ++q)
+	{
+#pragma unroll
+		for (int k = 0; k < (1 << q); ++k) T.w[(1 << q) - 1 + k] = tw[(sj << q) + k];
+	}
+}
+
+__device__ __forceinline__ mtw load_mtw(const tw_entry *__restrict__ twi) { mtw m; m.r = twi[0]; m.K = twi[1]; return m; }
+
+// Final-pass roots: f forward levels (level q holds 2^(3-f+q) distinct roots
+// per thread), the leaf level (8), and the f inverse levels.
+template <int LNL>
+struct ftw
+{
+	static constexpr int f = geo<LNL>::f;
+	static constexpr int nf = (f > 0) ? 8 - (8 >> f) : 1;
+	tw_entry fq[nf];
+	tw_entry last[8];
+	tw_entry iq[nf];
+	__device__ __forceinline__ tw_entry fwd(int q, int c) const { return fq[(8 >> f) * ((1 << q) - 1) + c]; }
+	__device__ __forceinline__ tw_entry leaf(int c) const { return last[c]; }
+	__device__ __forceinline__ tw_entry inv(int q, int c) const { return iq[(8 >> f) * ((1 << q) - 1) + c]; }
+};
+
+template <int LNL>
+struct ftw_lazy
+{
+	static constexpr int f = geo<LNL>::f, a = geo<LNL>::a;
+	const tw_entry *twf, *twi; uint32_t base, k0;
+	__device__ __forceinline__ tw_entry fwd(int q, int c) const { return twf[(base << (4 * a + q)) + (k0 >> (f + 1 - q)) + c]; }
+	__device__ __forceinline__ tw_entry leaf(int c) const { return twf[base * (geo<LNL>::n / 2) + (k0 >> 1) + c]; }
+	__device__ __forceinline__ tw_entry inv(int q, int c) const { return twi[(base << (4 * a + q)) + (k0 >> (f + 1 - q)) + c]; }
+};
+
+template <int LNL>
+__device__ __forceinline__ void load_ftw(ftw<LNL> &F, const tw_entry *__restrict__ twf, const tw_entry *__restrict__ twi,
+                                         uint32_t base, uint32_t tt)
+{
+	constexpr int f = geo<LNL>::f, a = geo<LNL>::a;
+	constexpr uint32_t n = geo<LNL>::n;
+	const uint32_t k0 = E * tt;
+#pragma unroll
+	for (int q = 0; q < f; ++q)
+	{
+		const uint32_t s = base << (4 * a + q);
+		const uint32_t off = (8 >> f) * ((1u << q) - 1), cnt = 16u >> (f + 1 - q);
+#pragma unroll
+		for (uint32_t c = 0; c < cnt; ++c)
+		{
+			F.fq[off + c] = twf[s + (k0 >> (f + 1 - q)) + c];
+			F.iq[off + c] = twi[s + (k0 >> (f + 1 - q)) + c];
+		}
+	}
+	const tw_entry *tl = twf + base * (n / 2) + (k0 >> 1);
+#pragma unroll
+	for (int z = 0; z < 8; ++z) F.last[z] = tl[z];
+}
+
 // ---- radix-16 DIF pass over the instance's levels 4L+1 .. 4L+4 -------------
 //
 // Value ranges (lazy butterflies, p < 2^30):
@@ -100,28 +193,22 @@ __device__ __forceinline__ void xchg(V (&u)[E], V *lds, uint32_t tt)
 //   the level-1 (Phi_6) forward split takes inputs in [0, 3p) (digits), the
 //   level-1 merge returns reduced values in [0, p).
 
-template <int LNL, int P, int L, bool PHI6>
-__device__ __forceinline__ void fwd_pass(uint32_t (&u)[E], const tw_entry *__restrict__ twf, uint32_t base, uint32_t tt)
+template <int P, bool LEVEL1, typename TW>
+__device__ __forceinline__ void fwd_pass(uint32_t (&u)[E], const TW &T)
 {
 	constexpr uint32_t p = kP[P];
-	constexpr uint32_t mm = (geo<LNL>::n >> (4 * L)) / 16;
-	const uint32_t j = (L == 0) ? 0u : tt / mm;
-	const uint32_t sj = (base << (4 * L)) + j;
 #pragma unroll
 	for (int q = 0; q < 4; ++q)
 	{
 		const int h = 8 >> q;
-		tw_entry w[8];
-#pragma unroll
-		for (int k = 0; k < (1 << q); ++k) w[k] = twf[(sj << q) + k];
 #pragma unroll
 		for (int v = 0; v < E; ++v)
 		{
 			if ((v & h) != 0) continue;
-			const tw_entry wv = w[v >> (4 - q)];
+			const tw_entry wv = T.get(q, v >> (4 - q));
 			const uint32_t x = red2p(u[v], p);
 			const uint32_t s = shoup_neg(u[v + h], wv.w, wv.wp, p);     // 2^32 - w y'
-			if (PHI6 && L == 0 && q == 0)
+			if (LEVEL1 && q == 0)
 			{   // level 1: mod X^(n/2) - r and X^(n/2) - r', r' = 1 - r:
 				// (x + r y, y + x - r y)
 				const uint32_t yx = red2p(red2p(u[v + h], p) + x, p);
@@ -137,38 +224,31 @@ __device__ __forceinline__ void fwd_pass(uint32_t (&u)[E], const tw_entry *__res
 	}
 }
 
-template <int LNL, int P, int L, bool PHI6>
-__device__ __forceinline__ void inv_pass(uint32_t (&u)[E], const tw_entry *__restrict__ twi, uint32_t base, uint32_t tt)
+template <int P, bool LEVEL1, typename TW>
+__device__ __forceinline__ void inv_pass(uint32_t (&u)[E], const TW &T, const mtw &M)
 {
 	constexpr uint32_t p = kP[P];
-	constexpr uint32_t mm = (geo<LNL>::n >> (4 * L)) / 16;
-	const uint32_t j = (L == 0) ? 0u : tt / mm;
-	const uint32_t sj = (base << (4 * L)) + j;
 #pragma unroll
 	for (int q = 3; q >= 0; --q)
 	{
 		const int h = 8 >> q;
-		if (PHI6 && L == 0 && q == 0)
+		if (LEVEL1 && q == 0)
 		{   // level-1 merge with the 2/n, 2^32 and (1-2r)^-1 scale folded into K
-			const tw_entry r = twi[0], K = twi[1];
 #pragma unroll
 			for (int v = 0; v < h; ++v)
 			{
 				const uint32_t x = u[v], y = u[v + h];
-				const uint32_t vh = shoup(x + y, r.w, r.wp, p);
-				u[v + h] = red1p(shoup(y + 2 * p - x, K.w, K.wp, p), p);
-				u[v] = red1p(shoup(x + 2 * p - vh, K.w, K.wp, p), p);
+				const uint32_t vh = shoup(x + y, M.r.w, M.r.wp, p);
+				u[v + h] = red1p(shoup(y + 2 * p - x, M.K.w, M.K.wp, p), p);
+				u[v] = red1p(shoup(x + 2 * p - vh, M.K.w, M.K.wp, p), p);
 			}
 			continue;
 		}
-		tw_entry w[8];
-#pragma unroll
-		for (int k = 0; k < (1 << q); ++k) w[k] = twi[(sj << q) + k];
 #pragma unroll
 		for (int v = 0; v < E; ++v)
 		{
 			if ((v & h) != 0) continue;
-			const tw_entry wv = w[v >> (4 - q)];
+			const tw_entry wv = T.get(q, v >> (4 - q));
 			const uint32_t x = u[v], y = u[v + h];
 			u[v] = red2p(x + y, p);
 			u[v + h] = shoup(x + 2 * p - y, wv.w, wv.wp, p);
@@ -178,25 +258,20 @@ __device__ __forceinline__ void inv_pass(uint32_t (&u)[E], const tw_entry *__res
 
 // ---- final pass: last f levels, pointwise square / product, f inverse levels
 
-template <int LNL, int P, bool MUL>
-__device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], const tw_entry *__restrict__ twf,
-                                           const tw_entry *__restrict__ twi, uint32_t base, uint32_t tt)
+template <int LNL, int P, bool MUL, typename FT>
+__device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], const FT &F)
 {
 	constexpr uint32_t p = kP[P], pq = kQ[P];
-	constexpr int f = geo<LNL>::f, a = geo<LNL>::a;
-	constexpr uint32_t n = geo<LNL>::n;
-	const uint32_t k0 = E * tt;
+	constexpr int f = geo<LNL>::f;
 #pragma unroll
 	for (int q = 0; q < f; ++q)
 	{
 		const int h = 1 << (f - q);
-		const uint32_t s = base << (4 * a + q);
-		const tw_entry *tq = twf + s + (k0 >> (f + 1 - q));
 #pragma unroll
 		for (int z = 0; z < E; ++z)
 		{
 			if ((z & h) != 0) continue;
-			const tw_entry w = tq[z >> (f + 1 - q)];
+			const tw_entry w = F.fwd(q, z >> (f + 1 - q));
 			{
 				const uint32_t x = red2p(u[z], p), sn = shoup_neg(u[z + h], w.w, w.wp, p);
 				u[z] = x - sn; u[z + h] = x + sn + 2 * p;
@@ -209,11 +284,10 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 		}
 	}
 	// last level: (u0 + u1 X)(v0 + v1 X) mod X^2 - w^2; products leave 2^-32
-	const tw_entry *tl = twf + base * (n / 2) + (k0 >> 1);
 #pragma unroll
 	for (int z = 0; z < E; z += 2)
 	{
-		const tw_entry w = tl[z >> 1];
+		const tw_entry w = F.leaf(z >> 1);
 		const uint32_t u0 = red2p(u[z], p), u1 = red2p(u[z + 1], p);
 		const uint32_t u1w = 0u - shoup_neg(u[z + 1], w.w, w.wp, p);             // w u1 in [0, 2p)
 		if (!MUL)
@@ -234,13 +308,11 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 	for (int q = f - 1; q >= 0; --q)
 	{
 		const int h = 1 << (f - q);
-		const uint32_t s = base << (4 * a + q);
-		const tw_entry *tq = twi + s + (k0 >> (f + 1 - q));
 #pragma unroll
 		for (int z = 0; z < E; ++z)
 		{
 			if ((z & h) != 0) continue;
-			const tw_entry w = tq[z >> (f + 1 - q)];
+			const tw_entry w = F.inv(q, z >> (f + 1 - q));
 			const uint32_t x = u[z], y = u[z + h];
 			u[z] = red2p(x + y, p);
 			u[z + h] = shoup(x + 2 * p - y, w.w, w.wp, p);
@@ -248,29 +320,79 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 	}
 }
 
-// forward passes 0..a-1, each followed by the transpose to the next layout
-// (the last one lands in layout a, the final-pass layout)
-template <int LNL, int P, int L, bool PHI6>
-__device__ __forceinline__ void fwd_chain(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twf, uint32_t base, uint32_t tt)
+// forward passes L..a-1, each followed by the transpose to the next layout
+// (the last one lands in layout a, the final-pass layout).  PF: the roots of
+// pass L+1 (and finally those of the final pass, into F) are loaded into
+// registers before pass L computes, hiding their latency at the price of live
+// VGPRs; otherwise each pass reads its roots where it uses them.
+template <int LNL, int P, int L, bool PHI6, bool PF>
+__device__ __forceinline__ void fwd_chain(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twf, const tw_entry *twi,
+                                          uint32_t base, uint32_t tt, const ptw &cur, ftw<LNL> &F)
 {
 	if constexpr (L < geo<LNL>::a)
 	{
-		fwd_pass<LNL, P, L, PHI6>(u, twf, base, tt);
+		constexpr bool last = (L + 1 == geo<LNL>::a);
+		ptw nxt;
+		if constexpr (PF)
+		{
+			if constexpr (!last) load_ptw<LNL, L + 1>(nxt, twf, base, tt);
+			else load_ftw<LNL>(F, twf, twi, base, tt);
+			fwd_pass<P, PHI6 && L == 0>(u, cur);
+		}
+		else fwd_pass<P, PHI6 && L == 0>(u, lazy_ptw<LNL, L>(twf, base, tt));
 		xchg<LNL, L, L + 1>(u, lds, tt);
-		fwd_chain<LNL, P, L + 1, PHI6>(u, lds, twf, base, tt);
+		if constexpr (!last) fwd_chain<LNL, P, L + 1, PHI6, PF>(u, lds, twf, twi, base, tt, nxt, F);
 	}
 }
 
+// Forward passes of one instance from layout 0.
+template <int LNL, int P, bool PHI6, bool PF>
+__device__ __forceinline__ void fwd_all(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twf, const tw_entry *twi,
+                                        uint32_t base, uint32_t tt, ftw<LNL> &F)
+{
+	static_assert(geo<LNL>::a > 0, "instances have at least one radix-16 pass");
+	ptw c0;
+	if constexpr (PF) load_ptw<LNL, 0>(c0, twf, base, tt);
+	fwd_chain<LNL, P, 0, PHI6, PF>(u, lds, twf, twi, base, tt, c0, F);
+}
+
 // inverse passes L..0, each preceded by the transpose from layout L+1
-template <int LNL, int P, int L, bool PHI6>
-__device__ __forceinline__ void inv_chain(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twi, uint32_t base, uint32_t tt)
+template <int LNL, int P, int L, bool PHI6, bool PF>
+__device__ __forceinline__ void inv_chain(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twi, uint32_t base, uint32_t tt,
+                                          const ptw &cur, const mtw &M)
 {
 	if constexpr (L >= 0)
 	{
+		ptw nxt;
+		if constexpr (PF && L > 0) load_ptw<LNL, L - 1>(nxt, twi, base, tt);
 		xchg<LNL, L + 1, L>(u, lds, tt);
-		inv_pass<LNL, P, L, PHI6>(u, twi, base, tt);
-		inv_chain<LNL, P, L - 1, PHI6>(u, lds, twi, base, tt);
+		if constexpr (PF) inv_pass<P, PHI6 && L == 0>(u, cur, M);
+		else inv_pass<P, PHI6 && L == 0>(u, lazy_ptw<LNL, L>(twi, base, tt), M);
+		if constexpr (L > 0) inv_chain<LNL, P, L - 1, PHI6, PF>(u, lds, twi, base, tt, nxt, M);
 	}
+}
+
+// Final pass then the inverse passes down to layout 0 (F: final-pass roots
+// loaded by fwd_all when PF).
+template <int LNL, int P, bool PHI6, bool MUL, bool PF>
+__device__ __forceinline__ void final_inv(uint32_t (&u)[E], uint32_t (&vv)[E], uint32_t *lds, const tw_entry *twf,
+                                          const tw_entry *twi, uint32_t base, uint32_t tt, const ftw<LNL> &F)
+{
+	constexpr int a = geo<LNL>::a;
+	mtw M;
+	if constexpr (PHI6) M = load_mtw(twi);
+	ptw c;
+	if constexpr (PF)
+	{
+		load_ptw<LNL, a - 1>(c, twi, base, tt);
+		final_pass<LNL, P, MUL>(u, vv, F);
+	}
+	else
+	{
+		ftw_lazy<LNL> FL; FL.twf = twf; FL.twi = twi; FL.base = base; FL.k0 = E * tt;
+		final_pass<LNL, P, MUL>(u, vv, FL);
+	}
+	inv_chain<LNL, P, a - 1, PHI6, PF>(u, lds, twi, base, tt, c, M);
 }
 
 // Garner CRT of the three engine residues to a signed 96-bit integer in

@@ -69,24 +69,29 @@ __host__ __device__ __forceinline__ uint32_t cslot(uint32_t g, uint32_t cpr, uin
 }
 
 template <int RL, int P, int L>
-__device__ __forceinline__ void col_fwd(uint32_t (&u)[E], uint32_t *X, const tw_entry *twf, uint32_t tt)
+__device__ __forceinline__ void col_fwd(uint32_t (&u)[E], uint32_t *X, const tw_entry *twf, uint32_t tt, const ptw &cur)
 {
-	fwd_pass<RL, P, L, true>(u, twf, 1u, tt);
+	ptw nxt;
+	if constexpr (L + 1 < RL / 4) load_ptw<RL, L + 1>(nxt, twf, 1u, tt);
+	fwd_pass<P, L == 0>(u, cur);
 	if constexpr (L + 1 < RL / 4)
 	{
 		xchg<RL, L, L + 1>(u, X, tt);
-		col_fwd<RL, P, L + 1>(u, X, twf, tt);
+		col_fwd<RL, P, L + 1>(u, X, twf, tt, nxt);
 	}
 }
 
 template <int RL, int P, int L>
-__device__ __forceinline__ void col_inv(uint32_t (&u)[E], uint32_t *X, const tw_entry *twi, uint32_t tt)
+__device__ __forceinline__ void col_inv(uint32_t (&u)[E], uint32_t *X, const tw_entry *twi, uint32_t tt, const ptw &cur,
+                                        const mtw &M)
 {
-	inv_pass<RL, P, L, true>(u, twi, 1u, tt);
+	ptw nxt;
+	if constexpr (L > 0) load_ptw<RL, L - 1>(nxt, twi, 1u, tt);
+	inv_pass<P, L == 0>(u, cur, M);
 	if constexpr (L > 0)
 	{
 		xchg<RL, L, L - 1>(u, X, tt);
-		col_inv<RL, P, L - 1>(u, X, twi, tt);
+		col_inv<RL, P, L - 1>(u, X, twi, tt, nxt, M);
 	}
 }
 
@@ -107,9 +112,11 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 		for (int q = 0; q < E; ++q) u[q] = from_signed(dr[loff<RL, 0>(q)], p);
 		const tw_entry *twf = tw + (size_t)(P * 2) * n;
 		uint32_t *Xc = X + cw * G::RP;
-		if (P == 0) col_fwd<RL, 0, 0>(u, Xc, twf, tt);
-		else if (P == 1) col_fwd<RL, 1, 0>(u, Xc, twf, tt);
-		else col_fwd<RL, 2, 0>(u, Xc, twf, tt);
+		ptw w0;
+		load_ptw<RL, 0>(w0, twf, 1u, tt);
+		if (P == 0) col_fwd<RL, 0, 0>(u, Xc, twf, tt, w0);
+		else if (P == 1) col_fwd<RL, 1, 0>(u, Xc, twf, tt, w0);
+		else col_fwd<RL, 2, 0>(u, Xc, twf, tt, w0);
 		uint32_t *out = res_lane + P * plane + c0 + cw;
 #pragma unroll
 		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, La>(tt, q) * C] = u[q];
@@ -167,21 +174,21 @@ __device__ __forceinline__ void rowB_prime(plain_args &A, uint32_t *lds, uint32_
 	uint32_t u[E], v[E];
 #pragma unroll
 	for (int q = 0; q < E; ++q) u[q] = row[t + q * T];
-	fwd_chain<CL, P, 0, false>(u, lds, twf, base, t);
+	ftw<CL> F;
+	fwd_all<CL, P, false, true>(u, lds, twf, twi, base, t, F);
 	if (MUL)
 	{
 		const uint32_t *row2 = A.res2 + P * plane + (size_t)lane * n + (size_t)r * C;
 #pragma unroll
 		for (int q = 0; q < E; ++q) v[q] = row2[t + q * T];
-		fwd_chain<CL, P, 0, false>(v, lds, twf, base, t);
+		fwd_all<CL, P, false, true>(v, lds, twf, twi, base, t, F);
 	}
 	else
 	{
 #pragma unroll
 		for (int q = 0; q < E; ++q) v[q] = 0;
 	}
-	final_pass<CL, P, MUL>(u, v, twf, twi, base, t);
-	inv_chain<CL, P, geo<CL>::a - 1, false>(u, lds, twi, base, t);
+	final_inv<CL, P, false, MUL, true>(u, v, lds, twf, twi, base, t, F);
 #pragma unroll
 	for (int q = 0; q < E; ++q) row[t + q * T] = u[q];
 }
@@ -212,21 +219,29 @@ __global__ __launch_bounds__((colgeo<LN, RL>::threads)) void k_colC(plain_args A
 	const uint32_t cw = t % G::W, tt = t / G::W;
 	const size_t plane = (size_t)A.lanes * n;
 	const uint32_t *res = A.res + (size_t)lane * n;
+	// every prime's residues are requested before any of them is transformed
+	uint32_t uu[NP][E];
 #pragma unroll
 	for (int P = 0; P < NP; ++P)
 	{
-		uint32_t u[E];
 		const uint32_t *in = res + P * plane + c0 + cw;
 #pragma unroll
-		for (int q = 0; q < E; ++q) u[q] = in[(size_t)elem<RL, La>(tt, q) * C];
+		for (int q = 0; q < E; ++q) uu[P][q] = in[(size_t)elem<RL, La>(tt, q) * C];
+	}
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+	{
 		const tw_entry *twi = A.tw + (size_t)(P * 2 + 1) * n;
 		uint32_t *Xc = X + cw * G::RP;
-		if (P == 0) col_inv<RL, 0, La>(u, Xc, twi, tt);
-		else if (P == 1) col_inv<RL, 1, La>(u, Xc, twi, tt);
-		else col_inv<RL, 2, La>(u, Xc, twi, tt);
+		ptw c;
+		load_ptw<RL, La>(c, twi, 1u, tt);
+		const mtw M = load_mtw(twi);
+		if (P == 0) col_inv<RL, 0, La>(uu[0], Xc, twi, tt, c, M);
+		else if (P == 1) col_inv<RL, 1, La>(uu[1], Xc, twi, tt, c, M);
+		else col_inv<RL, 2, La>(uu[NP - 1], Xc, twi, tt, c, M);
 		uint32_t *sw = S[P] + cw * G::RP + lbase<RL, 0>(tt);
 #pragma unroll
-		for (int q = 0; q < E; ++q) sw[loff<RL, 0>(q)] = u[q];
+		for (int q = 0; q < E; ++q) sw[loff<RL, 0>(q)] = uu[P][q];
 	}
 	__syncthreads();
 	// chunk role

@@ -40,20 +40,20 @@ __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[
 	const int32_t *dr = D + lbase<LN, 0>(t);
 #pragma unroll
 	for (int k = 0; k < E; ++k) u[k] = from_signed(dr[loff<LN, 0>(k)], p);
-	fwd_chain<LN, P, 0, true>(u, lds, twf, 1u, t);
+	ftw<LN> F;
+	fwd_all<LN, P, true, false>(u, lds, twf, twi, 1u, t, F);
 	if (MUL)
 	{
 #pragma unroll
 		for (int k = 0; k < E; ++k) v[k] = from_signed(yd[k], p);
-		fwd_chain<LN, P, 0, true>(v, lds, twf, 1u, t);
+		fwd_all<LN, P, true, false>(v, lds, twf, twi, 1u, t, F);
 	}
 	else
 	{
 #pragma unroll
 		for (int k = 0; k < E; ++k) v[k] = 0;
 	}
-	final_pass<LN, P, MUL>(u, v, twf, twi, 1u, t);
-	inv_chain<LN, P, geo<LN>::a - 1, true>(u, lds, twi, 1u, t);
+	final_inv<LN, P, true, MUL, false>(u, v, lds, twf, twi, 1u, t, F);
 #pragma unroll
 	for (int k = 0; k < E; ++k) out[k] = u[k];
 }
