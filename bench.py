@@ -53,15 +53,22 @@ def run_batched(ct, steps, warmup):
     return lanes * steps / (ms / 1e3), ms, i0, bs
 
 
-def run_plain(ct, ln, b, steps, warmup):
+def run_plain(ct, ln, b, steps, warmup, graphs=True):
+    """Plain multipass engine, one candidate: `warmup` untimed squarings, then
+    `steps` timed ones.  The warm-up also runs one run of the timed length, so
+    that the captured graph of that length exists (a long PRP test replays it
+    millions of times)."""
     from ctwin_amd.exponent import window_value
     n = 1 << ln
-    Ew, i0 = window_value(b, n, steps + warmup + 8)
+    Ew, i0 = window_value(b, n, 2 * steps + warmup + 8)
     t = ct.Transform(ln, 1, ct.KIND_PLAIN)
+    t.set_graphs(graphs)
     t.set_b([b]); t.set_exponents([Ew]); t.set(1)
     t.run(i0, i0 - warmup + 1)
-    t.timer_start()
     t.run(i0 - warmup, i0 - warmup - steps + 1)
+    i = i0 - warmup - steps
+    t.timer_start()
+    t.run(i, i - steps + 1)
     ms = t.timer_stop()
     t.close()
     return steps / (ms / 1e3), ms

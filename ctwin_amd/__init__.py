@@ -30,7 +30,8 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_run", "ct_prp_check", "ct_engine_timer_start", "ct_engine_timer_stop", "ct_engine_info",
            "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes",
            "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
-           "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli"]
+           "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
+           "ct_engine_set_graphs"]
 
 
 class CtError(RuntimeError):
@@ -87,6 +88,7 @@ def lib():
             "ct_server_stats": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u64), c.POINTER(u64)],
             "ct_engine_primes": [vp, c.POINTER(i32)],
             "ct_engine_moduli": [u32p],
+            "ct_engine_set_graphs": [vp, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -191,6 +193,10 @@ class Transform:
 
     def run(self, i_hi, i_lo=0, gl_block=0):
         _check(lib().ct_engine_run(self.h, int(i_hi), int(i_lo), int(gl_block)))
+
+    def set_graphs(self, on):
+        """Multipass engine: replay captured HIP graphs for runs (default) or launch each kernel."""
+        _check(lib().ct_engine_set_graphs(self.h, int(bool(on))))
 
     def set_primes(self, np_):
         """0: automatic (2 primes when the coefficient bound allows), 3: always 3."""

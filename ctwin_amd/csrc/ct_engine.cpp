@@ -108,6 +108,13 @@ int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind)
 	return CT_OK;
 }
 
+int ct_engine_set_graphs(ct_engine *e, int on)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	e->use_graphs = (on != 0);
+	return CT_OK;
+}
+
 int ct_engine_moduli(uint32_t *p)
 {
 	if (!p) return fail(CT_ERR_ARG, "null pointer");

@@ -35,6 +35,12 @@ struct ct_engine
 	uint64_t b_max = 0;               // largest b over the lanes
 	uint64_t state_bound = 1;         // bound on |digit| set or written into the registers
 	bool force3 = false;              // always use three primes
+	// captured squaring sequences of the multipass engine (hipGraph), keyed
+	// by what the captured kernel arguments depend on
+	struct graph_entry { int nsteps, np; bool first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
+	std::vector<graph_entry> graphs;
+	bool use_graphs = true;
+	int32_t *d_run = nullptr;         // {top bit index of the run, squarings done by replayed graphs}
 
 	int32_t *reg(uint32_t r) const { return d_x + (size_t)r * lanes * n; }
 	size_t reg_bytes() const { return (size_t)lanes * n * sizeof(int32_t); }

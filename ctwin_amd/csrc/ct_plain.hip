@@ -17,6 +17,7 @@
 // C 12 read, 4+0.5 write = 57 bytes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -39,9 +40,9 @@ struct plain_args
 	const uint32_t *bits;       // exponent bits or null
 	const uint32_t *mask;       // dup bits when bits is null (or null)
 	uint32_t bit_words;
-	int i;                      // exponent bit index of this squaring
-	const int32_t *lane_i;      // per-lane top bit index (or null); negative = idle lane
-	int step;                   // squarings already done in this run (bit = lane_i - step)
+	const int32_t *run;         // device {i0, iter}: top bit index of the run, squarings done by earlier graph replays
+	const int32_t *lane_i;      // per-lane top bit index (or null, then i0); negative = idle lane
+	int step;                   // squaring index within this launch sequence: bit = top - (iter + step)
 	uint32_t lanes;
 	garner_consts gc;
 };
@@ -252,7 +253,7 @@ __global__ __launch_bounds__((colgeo<LN, RL>::threads)) void k_colC(plain_args A
 	bool dup = false;
 	if (!no_dup)
 	{
-		const int bi = A.lane_i ? A.lane_i[lane] - A.step : A.i;
+		const int bi = (A.lane_i ? A.lane_i[lane] : A.run[0]) - (A.run[1] + A.step);
 		if (A.bits) dup = (A.bits[(size_t)lane * A.bit_words + (bi >> 5)] >> (bi & 31)) & 1u;
 		else if (A.mask) dup = (A.mask[lane >> 5] >> (lane & 31)) & 1u;
 	}
@@ -305,6 +306,11 @@ __global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, ui
 #pragma unroll
 	for (int v = 0; v < 4; ++v) p[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
 }
+
+// run control: {i0, iter} (one thread; kernel nodes only, so that the
+// sequence can be captured in a graph)
+__global__ void k_run_set(int32_t *run, int32_t i0) { run[0] = i0; run[1] = 0; }
+__global__ void k_run_add(int32_t *run, int32_t k) { run[1] += k; }
 
 // ---------------------------------------------------------------------------
 // host sequencing
@@ -384,13 +390,21 @@ int plain_alloc(ct_engine *e)
 	if (hipMalloc((void **)&e->d_res2, words * sizeof(uint32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc residues");
 	if (hipMalloc((void **)&e->d_carry, (size_t)e->lanes * (e->n / 16) * sizeof(int64_t)) != hipSuccess)
 		return fail(CT_ERR_NOMEM, "hipMalloc carries");
+	if (hipMalloc((void **)&e->d_run, 2 * sizeof(int32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc run");
 	return CT_OK;
+}
+
+static void drop_graphs(ct_engine *e)
+{
+	for (auto &g : e->graphs) hipGraphExecDestroy(g.exec);
+	e->graphs.clear();
 }
 
 void plain_free(ct_engine *e)
 {
-	hipFree(e->d_res); hipFree(e->d_res2); hipFree(e->d_carry);
-	e->d_res = e->d_res2 = nullptr; e->d_carry = nullptr;
+	drop_graphs(e);
+	hipFree(e->d_res); hipFree(e->d_res2); hipFree(e->d_carry); hipFree(e->d_run);
+	e->d_res = e->d_res2 = nullptr; e->d_carry = nullptr; e->d_run = nullptr;
 }
 
 static plain_args make_args(ct_engine *e)
@@ -398,7 +412,7 @@ static plain_args make_args(ct_engine *e)
 	plain_args A;
 	std::memset(&A, 0, sizeof(A));
 	A.res = e->d_res; A.res2 = e->d_res2; A.carry = e->d_carry;
-	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc;
+	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc; A.run = e->d_run;
 	return A;
 }
 
@@ -414,14 +428,73 @@ int plain_materialize(ct_engine *e)
 	return CT_OK;
 }
 
+// A run of squarings is launched as replays of captured graphs of up to
+// kGraphMax squarings (one graph per distinct length, first-carry state and
+// argument set; the last node advances run[1]) followed by eager launches.
+static constexpr int kGraphMax = 128;
+static constexpr size_t kGraphCache = 16;
+
+static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
+{
+	const bool fc = e->pending;
+	for (const auto &g : e->graphs)
+		if (g.nsteps == len && g.np == e->np && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
+		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
+	if (e->graphs.size() >= kGraphCache) drop_graphs(e);
+	hipGraph_t graph = nullptr;
+	HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+	hipError_t h = hipSuccess;
+	for (int j = 0; j < len && h == hipSuccess; ++j)
+	{
+		A.step = j;
+		A.carry_in = (j == 0 && !fc) ? nullptr : e->d_carry;
+		h = square_any((int)e->ln, e->np, A, e->lanes, e->st);
+	}
+	if (h == hipSuccess)
+	{
+		hipLaunchKernelGGL(k_run_add, dim3(1), dim3(1), 0, e->st, e->d_run, (int32_t)len);
+		h = hipGetLastError();
+	}
+	const hipError_t h2 = hipStreamEndCapture(e->st, &graph);
+	if (h != hipSuccess || h2 != hipSuccess)
+	{
+		if (graph) hipGraphDestroy(graph);
+		return hip_fail(h != hipSuccess ? h : h2, "graph capture of the squaring sequence");
+	}
+	hipGraphExec_t exec = nullptr;
+	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+	hipGraphDestroy(graph);
+	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
+	e->graphs.push_back({len, e->np, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
+	*out = exec;
+	return CT_OK;
+}
+
 int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words,
                 const int32_t *lane_i)
 {
 	plain_args A = make_args(e);
 	A.x = e->reg(0); A.bits = bits; A.bit_words = bit_words; A.mask = mask_words; A.lane_i = lane_i;
-	for (int s = 0; s < nsteps; ++s)
+	A.run = e->d_run;
+	hipLaunchKernelGGL(k_run_set, dim3(1), dim3(1), 0, e->st, e->d_run, (int32_t)i_hi);
+	HIPCHK(hipGetLastError());
+	int s = 0, in_graphs = 0;
+	if (e->use_graphs && nsteps > 1)
 	{
-		A.i = i_hi - s; A.step = s;
+		while (s < nsteps)
+		{
+			const int len = std::min(kGraphMax, nsteps - s);
+			hipGraphExec_t exec = nullptr;
+			int rc = graph_for(e, A, len, &exec);
+			if (rc) return rc;
+			HIPCHK(hipGraphLaunch(exec, e->st));
+			e->pending = true;
+			s += len; in_graphs += len;
+		}
+	}
+	for (; s < nsteps; ++s)
+	{
+		A.step = s - in_graphs;
 		A.carry_in = e->pending ? e->d_carry : nullptr;
 		HIPCHK(square_any((int)e->ln, e->np, A, e->lanes, e->st));
 		e->pending = true;

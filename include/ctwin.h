@@ -113,6 +113,13 @@ int ct_engine_timer_stop(ct_engine *e, float *ms);
 int ct_engine_set_primes(ct_engine *e, int np);
 int ct_engine_primes(const ct_engine *e, int *np);
 
+/* Multipass engine: launch runs of squarings as replays of captured HIP
+ * graphs (on = 1, the default) or as individual kernel launches (on = 0);
+ * the results are identical.  No counterpart in the reference, which enqueues
+ * every kernel (engine.h:387-419).  No effect on the resident kernels, which
+ * run a whole run in one launch. */
+int ct_engine_set_graphs(ct_engine *e, int on);
+
 /* Engine facts: n, lanes, kind actually used. */
 int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind);
 
