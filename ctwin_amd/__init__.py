@@ -31,7 +31,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes",
            "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
            "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
-           "ct_engine_set_graphs"]
+           "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning"]
 
 
 class CtError(RuntimeError):
@@ -89,6 +89,8 @@ def lib():
             "ct_engine_primes": [vp, c.POINTER(i32)],
             "ct_engine_moduli": [u32p],
             "ct_engine_set_graphs": [vp, i32],
+            "ct_engine_set_column_tile": [vp, u32],
+            "ct_engine_tuning": [vp, c.POINTER(u32), np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -193,6 +195,17 @@ class Transform:
 
     def run(self, i_hi, i_lo=0, gl_block=0):
         _check(lib().ct_engine_run(self.h, int(i_hi), int(i_lo), int(gl_block)))
+
+    def set_column_tile(self, w):
+        """Multipass engine: columns per workgroup of the column passes (16 or 32 for n >= 2^13)."""
+        _check(lib().ct_engine_set_column_tile(self.h, int(w)))
+
+    def tuning(self):
+        """(column tile, {resident, multipass W=16, W=32} ms per squaring; -1 = not timed)."""
+        w = ctypes.c_uint32()
+        ms = np.zeros(3, np.float32)
+        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ms))
+        return int(w.value), [float(x) for x in ms]
 
     def set_graphs(self, on):
         """Multipass engine: replay captured HIP graphs for runs (default) or launch each kernel."""

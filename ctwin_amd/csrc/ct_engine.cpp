@@ -6,7 +6,10 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/ctwin.h"
@@ -41,15 +44,11 @@ int ct::hip_fail(hipError_t e, const char *where)
 // ---------------------------------------------------------------------------
 // engine lifetime
 
-int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out)
+static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, uint32_t col_w, ct_engine **out)
 {
-	if (!out) return fail(CT_ERR_ARG, "out is null");
-	*out = nullptr;
-	if (log2_n < 5 || log2_n > (uint32_t)kMaxLog2N) return fail(CT_ERR_UNSUPPORTED, "log2_n must be in [5, 22]");
-	if (lanes == 0) return fail(CT_ERR_ARG, "lanes must be > 0");
-	if (csize != 0 && csize != 16) return fail(CT_ERR_UNSUPPORTED, "csize must be 0 or 16");
 	if (kind == CT_KIND_AUTO) kind = (log2_n <= 14) ? CT_KIND_RESIDENT : CT_KIND_PLAIN;
 	if (kind == CT_KIND_RESIDENT && log2_n > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need log2_n <= 14");
+	if (kind == CT_KIND_PLAIN && log2_n < 10) return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
 	if (kind != CT_KIND_RESIDENT && kind != CT_KIND_PLAIN) return fail(CT_ERR_ARG, "unknown engine kind");
 	int ndev = 0;
 	HIPCHK(hipGetDeviceCount(&ndev));
@@ -58,6 +57,7 @@ int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize
 
 	ct_engine *e = new ct_engine();
 	e->device = device; e->ln = log2_n; e->n = 1u << log2_n; e->lanes = lanes; e->csize = 16; e->kind = kind;
+	e->col_w = col_w;
 	e->gc = make_garner();
 	e->b.assign(lanes, 0);
 	const size_t reg_words = (size_t)lanes * e->n;
@@ -82,6 +82,132 @@ int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize
 	if (h == hipSuccess) h = hipMemset(e->d_x, 0, 3 * reg_words * sizeof(int32_t));
 	if (h != hipSuccess) { ct_engine_destroy(e); return hip_fail(h, "table upload"); }
 	*out = e;
+	return CT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner (the role of the CSIZE autotuner, transform.h:383-419): for a
+// transform length and lane count, times squarings of a synthetic state under
+// every compiled configuration -- resident kernels vs the multipass engine
+// (n <= 2^14; one resident workgroup per candidate loses to the whole-GPU
+// multipass launch when there are few candidates) and the multipass column
+// tile (16 or 32 columns per workgroup) -- and keeps the fastest.  Results are
+// cached per (device, log2 n, lanes) for the life of the process.
+
+namespace {
+
+struct tune_key { int device; uint32_t ln, lanes; int kind;
+	bool operator<(const tune_key &o) const
+	{ return std::tie(device, ln, lanes, kind) < std::tie(o.device, o.ln, o.lanes, o.kind); } };
+struct tune_val { int kind; uint32_t col_w; float ms[3]; };
+std::mutex g_tune_mu;
+std::map<tune_key, tune_val> g_tune;
+
+constexpr int kTuneWarm = 4, kTuneSteps = 24;
+
+// ms per squaring of engine e (3 primes, b = 2^30 + 9699690 i as in
+// transform.h:391 but below 2^31), or a negative value on failure.
+float time_squarings(ct_engine *e)
+{
+	std::vector<uint32_t> b(e->lanes);
+	for (uint32_t i = 0; i < e->lanes; ++i) b[i] = (1u << 30) + 9699690u * (i % 100);
+	if (ct_engine_set_b(e, b.data()) || ct_engine_set(e, 3)) return -1;
+	if (ct::steps(e, nullptr, 0, 0, kTuneWarm, nullptr)) return -1;
+	float ms = 0;
+	if (ct_engine_timer_start(e) || ct::steps(e, nullptr, 0, 0, kTuneSteps, nullptr) || ct_engine_timer_stop(e, &ms)) return -1;
+	return ms / kTuneSteps;
+}
+
+}   // namespace
+
+static int tune(int device, uint32_t ln, uint32_t lanes, int kind, tune_val *out)
+{
+	const tune_key key{device, ln, lanes, kind};
+	{
+		std::lock_guard<std::mutex> g(g_tune_mu);
+		auto it = g_tune.find(key);
+		if (it != g_tune.end()) { *out = it->second; return CT_OK; }
+	}
+	tune_val v{CT_KIND_AUTO, 0, {-1, -1, -1}};
+	float best = 1e30f;
+	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && ln <= 14);
+	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && ln >= 10);
+	if (try_res)
+	{
+		ct_engine *e = nullptr;
+		int rc = create_impl(device, ln, lanes, CT_KIND_RESIDENT, 0, &e);
+		if (rc) return rc;
+		v.ms[0] = time_squarings(e);
+		ct_engine_destroy(e);
+		if (v.ms[0] < 0) return fail(CT_ERR_HIP, "autotuner: timing the resident kernels failed");
+		if (v.ms[0] < best) { best = v.ms[0]; v.kind = CT_KIND_RESIDENT; v.col_w = 0; }
+	}
+	if (try_plain)
+	{
+		ct_engine *e = nullptr;
+		int rc = create_impl(device, ln, lanes, CT_KIND_PLAIN, 0, &e);
+		if (rc) return rc;
+		const uint32_t tiles[2] = {16, 32};
+		for (int t = 0; t < 2; ++t)
+		{
+			if (!plain_tile_ok((int)ln, tiles[t])) continue;
+			e->col_w = tiles[t];
+			const float ms = time_squarings(e);
+			if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
+			v.ms[1 + t] = ms;
+			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; }
+		}
+		if (!plain_tile_ok((int)ln, 16) && !plain_tile_ok((int)ln, 32))
+		{   // small n: one tile only
+			const float ms = time_squarings(e);
+			if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
+			v.ms[1] = ms;
+			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = plain_default_tile((int)ln); }
+		}
+		ct_engine_destroy(e);
+	}
+	if (v.kind == CT_KIND_AUTO) return fail(CT_ERR_UNSUPPORTED, "autotuner: no configuration for this size");
+	{
+		std::lock_guard<std::mutex> g(g_tune_mu);
+		g_tune[key] = v;
+	}
+	*out = v;
+	return CT_OK;
+}
+
+int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out)
+{
+	if (!out) return fail(CT_ERR_ARG, "out is null");
+	*out = nullptr;
+	if (log2_n < 5 || log2_n > (uint32_t)kMaxLog2N) return fail(CT_ERR_UNSUPPORTED, "log2_n must be in [5, 22]");
+	if (lanes == 0) return fail(CT_ERR_ARG, "lanes must be > 0");
+	if (csize != 0 && csize != 16) return fail(CT_ERR_UNSUPPORTED, "csize must be 0 or 16");
+	if (kind != CT_KIND_AUTO && kind != CT_KIND_RESIDENT && kind != CT_KIND_PLAIN) return fail(CT_ERR_ARG, "unknown engine kind");
+	if (kind == CT_KIND_RESIDENT && log2_n > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need log2_n <= 14");
+	if (kind == CT_KIND_PLAIN && log2_n < 10) return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
+	if (csize == 16) return create_impl(device, log2_n, lanes, kind, 0, out);
+	tune_val v;
+	int rc = tune(device, log2_n, lanes, kind, &v);
+	if (rc) return rc;
+	rc = create_impl(device, log2_n, lanes, v.kind, v.col_w, out);
+	if (rc == CT_OK) std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
+	return rc;
+}
+
+int ct_engine_set_column_tile(ct_engine *e, uint32_t w)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (e->kind != CT_KIND_PLAIN) return fail(CT_ERR_UNSUPPORTED, "column tiles belong to the multipass engine");
+	if (!plain_tile_ok((int)e->ln, w)) return fail(CT_ERR_UNSUPPORTED, "tile width not compiled for this length");
+	e->col_w = w;
+	return CT_OK;
+}
+
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, float *ms3)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (col_w) *col_w = (e->kind == CT_KIND_PLAIN) ? e->col_w : 0;
+	if (ms3) for (int i = 0; i < 3; ++i) ms3[i] = e->tune_ms[i];
 	return CT_OK;
 }
 

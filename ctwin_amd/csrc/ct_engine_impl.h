@@ -30,6 +30,8 @@ struct ct_engine
 	uint32_t *d_res2 = nullptr;       // second operand of a product
 	int64_t *d_carry = nullptr;       // [lanes][n/16] chunk carries
 	uint32_t rows_log2 = 0;           // column length R = 2^rows_log2
+	uint32_t col_w = 0;               // column-pass tile width (0 = default for the length)
+	float tune_ms[3] = {-1, -1, -1};  // autotuner timings: resident, multipass W=16, W=32 (ms per squaring)
 	bool pending = false;             // register 0 still owes the chunk carries (plain engine)
 	int np = 3;                       // primes per squaring: 2 when 3 n D^2 < p1 p2 / 2
 	uint64_t b_max = 0;               // largest b over the lanes
@@ -37,7 +39,7 @@ struct ct_engine
 	bool force3 = false;              // always use three primes
 	// captured squaring sequences of the multipass engine (hipGraph), keyed
 	// by what the captured kernel arguments depend on
-	struct graph_entry { int nsteps, np; bool first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
+	struct graph_entry { int nsteps, np; uint32_t col_w; bool first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
 	std::vector<graph_entry> graphs;
 	bool use_graphs = true;
 	int32_t *d_run = nullptr;         // {top bit index of the run, squarings done by replayed graphs}
@@ -64,5 +66,7 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
 int plain_materialize(ct_engine *e);
 void choose_primes(ct_engine *e);
+bool plain_tile_ok(int ln, uint32_t w);
+uint32_t plain_default_tile(int ln);
 
 }   // namespace ct

@@ -47,14 +47,15 @@ struct plain_args
 	garner_consts gc;
 };
 
-// Column-pass geometry: W adjacent columns per workgroup (64-byte row
-// segments for W = 16, 128-byte ones for W = 32 at large n).
-template <int LN, int RL>
+// Column-pass geometry: W_ adjacent columns per workgroup (the LDS tile;
+// 64-byte row segments for W = 16, 128-byte ones for W = 32).  The engine's
+// autotuner picks W per transform length.
+template <int LN, int RL, int W_>
 struct colgeo
 {
 	static constexpr uint32_t R = 1u << RL;
 	static constexpr uint32_t Tc = R / 16;                 // threads per column
-	static constexpr uint32_t W = (RL == 4) ? 64 : ((LN >= 19) ? 32 : 16);   // columns per workgroup
+	static constexpr uint32_t W = W_;                      // columns per workgroup
 	static constexpr uint32_t threads = W * Tc;            // = chunks per workgroup (R W / 16)
 	static constexpr uint32_t RP = R + (R >> 5);           // padded column stride in LDS
 	static constexpr int passes = RL / 4;
@@ -96,11 +97,11 @@ __device__ __forceinline__ void col_inv(uint32_t (&u)[E], uint32_t *X, const tw_
 	}
 }
 
-template <int LN, int RL, int NP>
+template <int LN, int RL, int NP, int W>
 __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, uint32_t *res_lane, const tw_entry *tw,
                                                uint32_t c0, uint32_t cw, uint32_t tt, size_t plane)
 {
-	using G = colgeo<LN, RL>;
+	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL;
 	constexpr int La = G::passes - 1;
 #pragma unroll
@@ -125,10 +126,10 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 }
 
 // A: chunk load (+ pending carry), column levels, residue planes
-template <int LN, int RL, int NP>
-__global__ __launch_bounds__((colgeo<LN, RL>::threads)) void k_colA(plain_args A, int operand)
+template <int LN, int RL, int NP, int W>
+__global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_args A, int operand)
 {
-	using G = colgeo<LN, RL>;
+	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
 	__shared__ int32_t D[G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
@@ -159,7 +160,7 @@ __global__ __launch_bounds__((colgeo<LN, RL>::threads)) void k_colA(plain_args A
 	}
 	__syncthreads();
 	const uint32_t cw = t % G::W, tt = t / G::W;
-	colA_prime_all<LN, RL, NP>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n);
+	colA_prime_all<LN, RL, NP, W>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n);
 }
 
 // B: row levels with the pointwise square / product, in place
@@ -207,10 +208,10 @@ __global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
 }
 
 // C: inverse column levels, CRT, normalize1
-template <int LN, int RL, int NP>
-__global__ __launch_bounds__((colgeo<LN, RL>::threads)) void k_colC(plain_args A, int no_dup)
+template <int LN, int RL, int NP, int W>
+__global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_args A, int no_dup)
 {
-	using G = colgeo<LN, RL>;
+	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
 	constexpr int La = G::passes - 1;
 	__shared__ uint32_t S[3][G::W * G::RP];
@@ -317,62 +318,82 @@ __global__ void k_run_add(int32_t *run, int32_t k) { run[1] += k; }
 
 static int rows_log2(int ln) { return (ln <= 12) ? 4 : 8; }
 
-template <int LN, int RL, int NP>
+// Column tiles compiled for a transform length: W = 64 with 16-row columns
+// (n <= 2^12); W = 16 or 32 with 256-row columns.
+static bool tile_ok(int ln, uint32_t w) { return (ln <= 12) ? (w == 64) : (w == 16 || w == 32); }
+uint32_t plain_default_tile(int ln) { return (ln <= 12) ? 64 : ((ln >= 19) ? 32 : 16); }
+
+template <int LN, int RL, int NP, int W>
 static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	using G = colgeo<LN, RL>;
+	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, A, 0);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, A, 0);
 	hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
-	hipLaunchKernelGGL((k_colC<LN, RL, NP>), gcol, dim3(G::threads), 0, st, A, 0);
+	hipLaunchKernelGGL((k_colC<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, A, 0);
 	return hipGetLastError();
 }
 
-template <int LN, int RL, int NP>
+template <int LN, int RL, int NP, int W>
 static hipError_t mul_ln(const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
 {
-	using G = colgeo<LN, RL>;
+	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, In, 0);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP>), gcol, dim3(G::threads), 0, st, In, 1);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, In, 0);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, In, 1);
 	hipLaunchKernelGGL((k_rowB<LN, RL, true>), dim3(R, NP, lanes), dim3(C / 16), 0, st, Out);
-	hipLaunchKernelGGL((k_colC<LN, RL, NP>), gcol, dim3(G::threads), 0, st, Out, 1);
+	hipLaunchKernelGGL((k_colC<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, Out, 1);
 	return hipGetLastError();
 }
 
-template <int LN, int RL>
+template <int LN, int RL, int W>
 static hipError_t square_np(int np, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	return (np == 2) ? square_ln<LN, RL, 2>(A, lanes, st) : square_ln<LN, RL, 3>(A, lanes, st);
+	return (np == 2) ? square_ln<LN, RL, 2, W>(A, lanes, st) : square_ln<LN, RL, 3, W>(A, lanes, st);
+}
+
+template <int LN, int RL, int W>
+static hipError_t mul_np(int np, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+{
+	return (np == 2) ? mul_ln<LN, RL, 2, W>(In, Out, lanes, st) : mul_ln<LN, RL, 3, W>(In, Out, lanes, st);
 }
 
 template <int LN, int RL>
-static hipError_t mul_np(int np, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+static hipError_t square_w(int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	return (np == 2) ? mul_ln<LN, RL, 2>(In, Out, lanes, st) : mul_ln<LN, RL, 3>(In, Out, lanes, st);
+	if constexpr (RL == 4) return square_np<LN, RL, 64>(np, A, lanes, st);
+	else return (w == 32) ? square_np<LN, RL, 32>(np, A, lanes, st) : square_np<LN, RL, 16>(np, A, lanes, st);
+}
+
+template <int LN, int RL>
+static hipError_t mul_w(int np, uint32_t w, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+{
+	if constexpr (RL == 4) return mul_np<LN, RL, 64>(np, In, Out, lanes, st);
+	else return (w == 32) ? mul_np<LN, RL, 32>(np, In, Out, lanes, st) : mul_np<LN, RL, 16>(np, In, Out, lanes, st);
 }
 
 #define CT_PLAIN_SIZES(X) X(10, 4) X(11, 4) X(12, 4) X(13, 8) X(14, 8) X(15, 8) X(16, 8) X(17, 8) \
 	X(18, 8) X(19, 8) X(20, 8) X(21, 8) X(22, 8)
 
-static hipError_t square_any(int ln, int np, const plain_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t square_any(int ln, int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
 	switch (ln)
 	{
-#define CT_SQ_CASE(L_, R_) case L_: return square_np<L_, R_>(np, A, lanes, st);
+#define CT_SQ_CASE(L_, R_) case L_: return square_w<L_, R_>(np, w, A, lanes, st);
 	CT_PLAIN_SIZES(CT_SQ_CASE)
 #undef CT_SQ_CASE
 	default: return hipErrorInvalidValue;
 	}
 }
 
-static hipError_t mul_any(int ln, int np, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+static hipError_t mul_any(int ln, int np, uint32_t w, const plain_args &In, const plain_args &Out, uint32_t lanes,
+                          hipStream_t st)
 {
 	switch (ln)
 	{
-#define CT_MUL_CASE(L_, R_) case L_: return mul_np<L_, R_>(np, In, Out, lanes, st);
+#define CT_MUL_CASE(L_, R_) case L_: return mul_w<L_, R_>(np, w, In, Out, lanes, st);
 	CT_PLAIN_SIZES(CT_MUL_CASE)
 #undef CT_MUL_CASE
 	default: return hipErrorInvalidValue;
@@ -381,10 +402,13 @@ static hipError_t mul_any(int ln, int np, const plain_args &In, const plain_args
 
 #define HIPCHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) return hip_fail(_e, #call); } while (0)
 
+bool plain_tile_ok(int ln, uint32_t w) { return tile_ok(ln, w); }
+
 int plain_alloc(ct_engine *e)
 {
 	if (e->ln < 10 || e->ln > 22) return fail(CT_ERR_UNSUPPORTED, "plain engine needs 10 <= log2_n <= 22");
 	e->rows_log2 = (uint32_t)rows_log2((int)e->ln);
+	if (!tile_ok((int)e->ln, e->col_w)) e->col_w = plain_default_tile((int)e->ln);
 	const size_t words = (size_t)3 * e->lanes * e->n;
 	if (hipMalloc((void **)&e->d_res, words * sizeof(uint32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc residues");
 	if (hipMalloc((void **)&e->d_res2, words * sizeof(uint32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc residues");
@@ -438,7 +462,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 {
 	const bool fc = e->pending;
 	for (const auto &g : e->graphs)
-		if (g.nsteps == len && g.np == e->np && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
+		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
 		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
 	if (e->graphs.size() >= kGraphCache) drop_graphs(e);
 	hipGraph_t graph = nullptr;
@@ -448,7 +472,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 	{
 		A.step = j;
 		A.carry_in = (j == 0 && !fc) ? nullptr : e->d_carry;
-		h = square_any((int)e->ln, e->np, A, e->lanes, e->st);
+		h = square_any((int)e->ln, e->np, e->col_w, A, e->lanes, e->st);
 	}
 	if (h == hipSuccess)
 	{
@@ -465,7 +489,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	hipGraphDestroy(graph);
 	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
-	e->graphs.push_back({len, e->np, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
+	e->graphs.push_back({len, e->np, e->col_w, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
 	*out = exec;
 	return CT_OK;
 }
@@ -496,7 +520,7 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 	{
 		A.step = s - in_graphs;
 		A.carry_in = e->pending ? e->d_carry : nullptr;
-		HIPCHK(square_any((int)e->ln, e->np, A, e->lanes, e->st));
+		HIPCHK(square_any((int)e->ln, e->np, e->col_w, A, e->lanes, e->st));
 		e->pending = true;
 	}
 	return CT_OK;
@@ -513,7 +537,7 @@ int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
 	In.x = e->reg(a); In.xb = e->reg(b);
 	plain_args Out = make_args(e);
 	Out.x = e->reg(dst);
-	HIPCHK(mul_any((int)e->ln, e->np, In, Out, e->lanes, e->st));
+	HIPCHK(mul_any((int)e->ln, e->np, e->col_w, In, Out, e->lanes, e->st));
 	// materialise dst at once (only register 0 may carry pending carries)
 	const size_t total = (size_t)e->lanes * (e->n / 16);
 	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,

@@ -95,11 +95,11 @@ bool exponent_window(uint32_t b, uint32_t n, uint32_t k, mpz_t w)
 	return ok;
 }
 
-uint64_t class_bytes(uint32_t ln, uint32_t slots, uint32_t words)
+uint64_t class_bytes(uint32_t ln, uint32_t slots, uint32_t words, bool multipass)
 {
 	const uint64_t n = 1ull << ln;
 	uint64_t b = 3 * (uint64_t)slots * n * 4 + 48 * n + (uint64_t)slots * (words * 4 + 64);
-	if (ln > 14) b += 2 * 3 * (uint64_t)slots * n * 4 + (uint64_t)slots * (n / 16) * 8;
+	if (multipass) b += 2 * 3 * (uint64_t)slots * n * 4 + (uint64_t)slots * (n / 16) * 8;
 	return b;
 }
 
@@ -157,11 +157,13 @@ int ct_server_submit(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uin
 static int make_class(ct_server *s, uint32_t ln, size_class **out)
 {
 	const uint32_t words = (s->max_steps + 31) / 32 + 1;
-	const uint64_t bytes = class_bytes(ln, s->slots, words);
-	if (s->used + bytes > s->budget) return CT_ERR_NOMEM;
+	// admission is checked against the larger (multipass) footprint; the
+	// autotuned engine kind then fixes what is charged
+	if (s->used + class_bytes(ln, s->slots, words, true) > s->budget) return CT_ERR_NOMEM;
 	std::unique_ptr<size_class> c(new size_class());
-	int rc = ct_engine_create(s->device, ln, s->slots, 16, CT_KIND_AUTO, &c->e);
+	int rc = ct_engine_create(s->device, ln, s->slots, 0, CT_KIND_AUTO, &c->e);
 	if (rc) return rc;
+	const uint64_t bytes = class_bytes(ln, s->slots, words, c->e->kind == CT_KIND_PLAIN);
 	c->ln = ln; c->words = words; c->bytes = bytes;
 	c->lanes.resize(s->slots);
 	c->lane_i.assign(s->slots, -1);

@@ -40,10 +40,26 @@ typedef struct ct_engine ct_engine;
 /* Replaces engine(platform, d, verbose) + transform(size, engine, isBoinc,
  * csize) (cyclo/src/engine.h:40, cyclo/src/transform.h:380-433): device
  * ordinal, log2 n (5..22; resident kernels to 14, multipass from 10), number of candidates ("lanes"; the reference
- * fixes VSIZE = 64, engine.h:19), carry chunk csize (0 or 16; the reference
- * autotunes 8..64, transform.h:383-419; the carried digits depend on it, the
- * residues do not), engine kind.  Allocates 3 digit registers per lane. */
+ * fixes VSIZE = 64, engine.h:19), carry chunk csize, engine kind.  The
+ * carry chunk is 16 digits (the carried digits depend on it, the residues do
+ * not).  csize = 16 builds the engine as asked (AUTO: resident kernels for
+ * n <= 2^14, else multipass).  csize = 0 runs the autotuner first, as the
+ * reference does for csize 0 (transform.h:383-419, which times CSIZE 8..64):
+ * it times squarings under each compiled configuration -- resident kernels
+ * vs the multipass engine when kind is AUTO and n <= 2^14, and the multipass
+ * column tile -- keeps the fastest, and caches the choice per (device, n,
+ * lanes, kind) for the process.  Allocates 3 digit registers per lane. */
 int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out);
+
+/* Multipass engine: columns per workgroup of the column passes (16 or 32 for
+ * n >= 2^13; 64 below).  Results do not depend on it.  No counterpart in the
+ * reference (its work-group geometry is fixed per kernel, kernel.cl). */
+int ct_engine_set_column_tile(ct_engine *e, uint32_t w);
+
+/* What the autotuner chose and measured: the column tile (0 for resident
+ * kernels) and ms per squaring for {resident, multipass W=16, multipass W=32}
+ * (-1 where not timed or when the engine was not tuned). */
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, float *ms3);
 int ct_engine_destroy(ct_engine *e);
 
 /* transform::set_b (transform.h:472-486): b[lanes], 32 < b < 2^31. */

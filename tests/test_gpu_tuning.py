@@ -1,0 +1,61 @@
+"""Autotuner (csize = 0 at engine creation) and the multipass column tile:
+every configuration gives the same carried digits."""
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ct = pytest.importorskip("ctwin_amd")
+
+
+def _run(t, bs, n, steps, seed):
+    rng = random.Random(seed)
+    E = [b ** n - b ** (n // 2) for b in bs] if n <= 1 << 16 else None
+    if E is None:
+        import oracle_lib as O
+        E = [top << s for top, s in (O.top_bits_of_exponent(b, n, 128) for b in bs)]
+    i0 = max(e.bit_length() for e in E) - 1
+    t.set_b(bs); t.set_exponents(E); t.set(1)
+    t.run(i0, i0 - steps + 1)
+    return t.read_state(0)
+
+
+@pytest.mark.parametrize("ln,lanes", [(12, 1), (14, 1), (14, 32), (17, 2)])
+def test_tuned_engine_matches_untuned(ln, lanes):
+    n = 1 << ln
+    rng = random.Random(800 + ln + lanes)
+    bs = [rng.randrange(1 << 17, 1 << 20) for _ in range(lanes)]
+    print(f"\n[case] tuned ln={ln} lanes={lanes}", flush=True)
+    tuned = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
+    w, ms = tuned.tuning()
+    timed = [m for m in ms if m > 0]
+    assert timed, ms
+    # the chosen configuration is the fastest one timed
+    best = min(timed)
+    if tuned.kind == ct.KIND_RESIDENT:
+        assert ms[0] == best and w == 0
+    else:
+        assert w in (16, 32, 64) and (ms[1] == best or ms[2] == best)
+    ref = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=16)
+    a = _run(tuned, bs, n, 40, 1)
+    b = _run(ref, bs, n, 40, 1)
+    assert np.array_equal(a, b)
+    # second creation hits the cache: same choice, same timings
+    again = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
+    assert again.kind == tuned.kind and again.tuning() == tuned.tuning()
+
+
+@pytest.mark.parametrize("ln", [13, 16, 20])
+def test_column_tiles_give_identical_digits(ln):
+    n = 1 << ln
+    rng = random.Random(900 + ln)
+    bs = [rng.randrange(1 << 16, 1 << 18)]
+    print(f"\n[case] tiles ln={ln}", flush=True)
+    out = []
+    for w in (16, 32):
+        t = ct.Transform(ln, 1, ct.KIND_PLAIN)
+        t.set_column_tile(w)
+        assert t.tuning()[0] == w
+        out.append(_run(t, bs, n, 30, 2))
+    assert np.array_equal(out[0], out[1])
