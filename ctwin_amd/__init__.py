@@ -31,7 +31,15 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes",
            "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
            "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
-           "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning"]
+           "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
+           "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit"]
+
+INTERRUPTED = 1
+
+
+class CycloStats(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_uint64), ("prps", ctypes.c_uint64), ("squarings", ctypes.c_uint64),
+                ("elapsed", ctypes.c_double), ("resumed", ctypes.c_int)]
 
 
 class CtError(RuntimeError):
@@ -90,12 +98,21 @@ def lib():
             "ct_engine_moduli": [u32p],
             "ct_engine_set_graphs": [vp, i32],
             "ct_engine_set_column_tile": [vp, u32],
+            "ct_sieve_save": [c.c_char_p, i32, u64, u64, u64, u64, u8p, i32],
+            "ct_sieve_load": [c.c_char_p, i32, u64, u64, c.POINTER(u64), c.POINTER(u64), u8p],
+            "ct_context_save": [vp, c.c_char_p, i32, i32, c.c_double],
+            "ct_context_load": [vp, c.c_char_p, c.POINTER(i32), c.POINTER(i32), c.POINTER(c.c_double)],
+            "ct_cyclo_check_file": [i32, u32, c.c_char_p, c.c_char_p, u32, c.c_double, u64, c.POINTER(CycloStats)],
             "ct_engine_tuning": [vp, c.POINTER(u32), np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
         }
         for name, args in sig.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = c.c_int
+        L.ct_crc32.argtypes = [u32, c.c_char_p, c.c_size_t]
+        L.ct_crc32.restype = u32
+        L.ct_cyclo_quit.argtypes = []
+        L.ct_cyclo_quit.restype = None
         L.ct_last_error.argtypes = []
         L.ct_last_error.restype = c.c_char_p
         _lib = L
@@ -207,6 +224,16 @@ class Transform:
         _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ms))
         return int(w.value), [float(x) for x in ms]
 
+    def context_save(self, path, k, i, elapsed=0.0):
+        """Checkpoint registers 0 and 1 in the reference's cyclo.ctx format."""
+        _check(lib().ct_context_save(self.h, os.fsencode(path), int(k), int(i), float(elapsed)))
+
+    def context_load(self, path):
+        """Restore registers 0 and 1 from a cyclo.ctx file (or its .old); returns (k, i, elapsed)."""
+        k, i, el = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().ct_context_load(self.h, os.fsencode(path), ctypes.byref(k), ctypes.byref(i), ctypes.byref(el)))
+        return k.value, i.value, el.value
+
     def set_graphs(self, on):
         """Multipass engine: replay captured HIP graphs for runs (default) or launch each kernel."""
         _check(lib().ct_engine_set_graphs(self.h, int(bool(on))))
@@ -314,6 +341,40 @@ def sieve_factors(m, p_min, p_max, mode, device=0, cap=1 << 20):
         _check(rc)
         out = pairs[:2 * cnt.value].reshape(-1, 2)
         return out[np.lexsort((out[:, 1], out[:, 0]))]
+
+
+def crc32(data, crc=0):
+    """CRC-32 (IEEE) of bytes, continued from crc."""
+    return int(lib().ct_crc32(crc, bytes(data), len(data)))
+
+
+def sieve_save(directory, m, b_min, b_max, p_min_pos, p_min_neg, struck, write_cand=True):
+    """Sieve::write: sv_<m>_<b_min>_<b_max>.dat (and cand_*.txt) in directory."""
+    struck = np.ascontiguousarray(struck, np.uint8)
+    _check(lib().ct_sieve_save(os.fsencode(directory), m, b_min, b_max, p_min_pos, p_min_neg, struck, int(write_cand)))
+
+
+def sieve_load(directory, m, b_min, b_max):
+    """Sieve::read: (struck bytes, p_min_pos, p_min_neg) from sv_<m>_<b_min>_<b_max>.dat."""
+    struck = np.zeros(b_max - b_min + 1, np.uint8)
+    pp, pn = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(lib().ct_sieve_load(os.fsencode(directory), m, b_min, b_max, ctypes.byref(pp), ctypes.byref(pn), struck))
+    return struck, pp.value, pn.value
+
+
+def cyclo_check_file(log2_n, cand_file, work_dir, lanes=64, checkpoint_s=600.0, max_steps=0, device=0):
+    """cyclo::check(filename) on the device: returns (status, stats dict); status
+    0 = done, INTERRUPTED = stopped with cyclo.ctx saved."""
+    st = CycloStats()
+    rc = lib().ct_cyclo_check_file(device, log2_n, os.fsencode(cand_file), os.fsencode(work_dir), lanes,
+                                   float(checkpoint_s), int(max_steps), ctypes.byref(st))
+    if rc not in (0, INTERRUPTED):
+        _check(rc)
+    return rc, {f: getattr(st, f) for f, _ in CycloStats._fields_}
+
+
+def cyclo_quit():
+    lib().ct_cyclo_quit()
 
 
 def survivors(struck, b_min):

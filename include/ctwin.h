@@ -27,7 +27,8 @@ typedef enum
 	CT_ERR_UNSUPPORTED = -4,  /* size outside the engine's range */
 	CT_ERR_GETINT = -5,       /* canonical read-out failed ("getInt failed", transform.h:589) */
 	CT_ERR_COMPUTE = -6,      /* residue 0 ("computation failed", transform.h:611 / cyclo.h:369) */
-	CT_ERR_GERBICZ = -7       /* Gerbicz-Li mismatch ("Gerbicz failed", cyclo.h:377) */
+	CT_ERR_GERBICZ = -7,      /* Gerbicz-Li mismatch ("Gerbicz failed", cyclo.h:377) */
+	CT_INTERRUPTED = 1        /* not an error: a batch run stopped on request, checkpoint saved */
 } ct_status;
 
 typedef struct ct_engine ct_engine;
@@ -193,6 +194,58 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
  * p_max < 2^63. */
 int ct_sieve_factors(int device, int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap,
                      uint64_t *count);
+
+/* ---------------- files and the batch driver ---------------- */
+
+/* CRC-32 (IEEE 802.3, reflected) continued over len more bytes; the files
+ * below end with ~crc ^ 0xa23777ac of all preceding bytes (file.h:80-121,
+ * ctsieve.cpp:497-529). */
+uint32_t ct_crc32(uint32_t crc, const void *buf, size_t len);
+
+/* Sieve::write / Sieve::read (sieve/CPU/ctsieve.cpp:551-629): the state file
+ * <dir>/sv_<m>_<b_min>_<b_max>.dat = [p_min_pos u64][p_min_neg u64][count u64]
+ * [index u32 of every unstruck b - b_min][crc], the previous one rotated to
+ * .old; with write_cand, also cand_<m>_<b_min>_<b_max>.txt (b per line,
+ * ascending).  struck[b - b_min] as in ct_sieve_window; dir may be NULL. */
+int ct_sieve_save(const char *dir, int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_min_neg,
+                  const uint8_t *struck, int write_cand);
+int ct_sieve_load(const char *dir, int m, uint64_t b_min, uint64_t b_max, uint64_t *p_min_pos, uint64_t *p_min_neg,
+                  uint8_t *struck);
+
+/* cyclo::saveContext / readContext (cyclo.h:107-176, transform.h:445-468):
+ * [int32 version 1][int32 k][int32 i][double elapsed][registers 0 and 1 as the
+ * reference's device buffers: x12 (residues of each carried digit modulo the
+ * reference primes P1, P2; index (reg n + j) lanes + lane) then x3 (modulo
+ * P3)][crc]; written via path.new with the previous file rotated to
+ * path.old; load falls back to path.old.  With 64 lanes the file is the
+ * reference's own cyclo.ctx (k = batch, i = next exponent bit). */
+int ct_context_save(ct_engine *e, const char *path, int k, int i, double elapsed);
+int ct_context_load(ct_engine *e, const char *path, int *k, int *i, double *elapsed);
+
+typedef struct
+{
+	uint64_t candidates;   /* distinct b tested in this call */
+	uint64_t prps;         /* probable primes found */
+	uint64_t squarings;    /* PRP-loop squarings run in this call (per lane) */
+	double elapsed;        /* seconds, including the time restored from a checkpoint */
+	int resumed;           /* 1 if the call resumed from cyclo.ctx */
+} ct_cyclo_stats;
+
+/* cyclo::check(filename) (cyclo.h:414-485): base-2 PRP with Gerbicz-Li of
+ * b^(2^log2_n) - b^(2^(log2_n - 1)) + 1 for every b of cand_file (one per line),
+ * lanes at a time (the reference: 64), appending
+ * "<b>^<n> - <b>^<n/2> + 1 is a probable prime." or "... is composite. RES64:
+ * <16 hex>" to <work_dir>/result_c.txt and probable primes to
+ * <work_dir>/prp_<log2_n>.txt.  Checkpoints <work_dir>/cyclo.ctx every
+ * checkpoint_s seconds at Gerbicz-Li points and resumes from it.  Stops with
+ * the checkpoint saved and returns CT_INTERRUPTED after ct_cyclo_quit() or
+ * once max_steps squarings ran in this call (0 = no limit).  On completion the
+ * checkpoint files are removed.  stats may be NULL. */
+int ct_cyclo_check_file(int device, uint32_t log2_n, const char *cand_file, const char *work_dir, uint32_t lanes,
+                        double checkpoint_s, uint64_t max_steps, ct_cyclo_stats *stats);
+/* Asks a running ct_cyclo_check_file to stop at its next segment boundary
+ * (async-signal-safe; the role of the reference's SIGINT handler). */
+void ct_cyclo_quit(void);
 
 /* Last error text of this thread (static storage). */
 const char *ct_last_error(void);

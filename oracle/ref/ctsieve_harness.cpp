@@ -3,7 +3,11 @@
 // hard-coded ini bounds of Sieve::check (ctsieve.cpp:800-801), and writes the
 // survivors (b in ascending order, one per line: the cand_*.txt format).
 //
-// usage: ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt
+// usage: ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt [dir]
+//
+// With dir, the reference's own Sieve::write(true) then writes its state file
+// sv_<m>_<b_min>_<b_max>.dat and cand_<m>_<b_min>_<b_max>.txt into dir (the
+// byte format fixtures of the file tests).
 //
 // The standard headers the reference includes are included first so that the
 // access override below only applies to the reference's own class.
@@ -21,6 +25,7 @@
 #include <cstdlib>
 #include <sys/stat.h>
 #include <signal.h>
+#include <unistd.h>
 
 #define main ctsieve_reference_main
 #define private public
@@ -30,7 +35,7 @@
 
 int main(int argc, char **argv)
 {
-	if (argc != 7) { std::cerr << "usage: ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt\n"; return 1; }
+	if (argc != 7 && argc != 8) { std::cerr << "usage: ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt [dir]\n"; return 1; }
 	const int m = std::atoi(argv[1]);
 	const uint64_t b_min = std::stoull(argv[2]), b_max = std::stoull(argv[3]);
 	const uint64_t p_max_pos = std::stoull(argv[4]), p_max_neg = std::stoull(argv[5]);
@@ -44,5 +49,11 @@ int main(int argc, char **argv)
 	catch (const std::runtime_error &e) { std::cerr << e.what() << std::endl; return 2; }
 	std::ofstream out(argv[6]);
 	for (size_t i = 0; i < s._bsieve.size(); ++i) if (!s._bsieve[i]) out << b_min + i << "\n";
+	out.close();
+	if (argc == 8)
+	{
+		if (chdir(argv[7]) != 0) { std::cerr << "cannot enter " << argv[7] << std::endl; return 3; }
+		s.write(true);
+	}
 	return 0;
 }

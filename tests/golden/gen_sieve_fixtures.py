@@ -7,7 +7,11 @@ keep the full ascending list (delta-encoded), the full window keeps the count,
 the SHA-256 of the cand_*.txt text ("b\\n" per survivor) and the first and
 last survivors.
 
-usage: python tests/golden/gen_sieve_fixtures.py [--full]
+With --files it also keeps the reference's own state and candidate files
+(Sieve::write(true)) of one tiny window under tests/golden/sieve_files/, the
+byte-format fixtures of the file tests.
+
+usage: python tests/golden/gen_sieve_fixtures.py [--full] [--files]
 """
 import hashlib
 import json
@@ -26,6 +30,7 @@ SMALL = [
     {"name": "m12_window64k", "m": 12, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 16) - 1,
      "p_max_pos": 1000000000, "p_max_neg": 100000000},
 ]
+FILES = {"m": 8, "b_min": 2, "b_max": 3001, "p_max_pos": 100000, "p_max_neg": 100000}
 FULL = {"name": "m12_config2", "m": 12, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 24) - 1,
         "p_max_pos": 1 << 32, "p_max_neg": 1 << 32}
 
@@ -51,6 +56,21 @@ def main():
         print(c["name"], len(surv), flush=True)
     json.dump({"source": "sieve/CPU/ctsieve.cpp check_pos + check_neg from the ini start",
                "cases": cases}, open(os.path.join(HERE, "sieve_small.json"), "w"))
+    if "--files" in sys.argv:
+        dst = os.path.join(HERE, "sieve_files")
+        os.makedirs(dst, exist_ok=True)
+        c = FILES
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref")])
+        with tempfile.TemporaryDirectory() as d:
+            subprocess.check_call([REF, str(c["m"]), str(c["b_min"]), str(c["b_max"]), str(c["p_max_pos"]),
+                                   str(c["p_max_neg"]), os.path.join(d, "s.txt"), d], stdout=subprocess.DEVNULL)
+            stem = f"_{c['m']}_{c['b_min']}_{c['b_max']}"
+            for name in ("sv" + stem + ".dat", "cand" + stem + ".txt"):
+                with open(os.path.join(d, name), "rb") as f, open(os.path.join(dst, name), "wb") as g:
+                    g.write(f.read())
+        json.dump(dict(FILES, source="sieve/CPU/ctsieve.cpp Sieve::write(true) after check_pos + check_neg"),
+                  open(os.path.join(dst, "case.json"), "w"))
+        print("files", stem)
     if "--full" in sys.argv:
         surv, h = run(FULL)
         json.dump(dict(FULL, source="sieve/CPU/ctsieve.cpp check_pos + check_neg from the ini start",
