@@ -43,14 +43,16 @@ def run_batched(ct, steps, warmup):
     bs = [rng.randrange(1 << 18, 1 << 19) for _ in range(lanes)]
     E = [exponent(b, n) for b in bs]
     i0 = max(e.bit_length() for e in E) - 1
-    t = ct.Transform(ln, lanes, ct.KIND_RESIDENT)
+    # configuration chosen by the autotuner (timed before this run, outside the timed region)
+    t = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
     t.set_b(bs); t.set_exponents(E); t.set(1)
     t.run(i0, i0 - warmup + 1)
     t.timer_start()
     t.run(i0 - warmup, i0 - warmup - steps + 1)
     ms = t.timer_stop()
+    kind, tile = t.kind, t.tuning()
     t.close()
-    return lanes * steps / (ms / 1e3), ms, i0, bs
+    return lanes * steps / (ms / 1e3), ms, i0, bs, kind, tile
 
 
 def run_plain(ct, ln, b, steps, warmup, graphs=True):
@@ -92,20 +94,38 @@ def run_sieve_config2(ct):
     return out
 
 
-def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30):
+def sieve_pool(ct, m, bits, cache):
+    """Sieve survivors (both twin members, p < 2^26) of the window
+    [2^(bits-1), 2^(bits-1) + 2^16) for n = 2^m digits: the b values the served
+    stream draws from."""
+    key = (m, bits)
+    if key not in cache:
+        lo = 1 << (bits - 1)
+        hi = lo + min(1 << 16, lo) - 1
+        st = ct.sieve_window(m, lo, hi, 1 << 26, 1 << 26)
+        cache[key] = [int(b) for b in ct.survivors(st, lo)]
+    return cache[key]
+
+
+def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30, graphs=True):
     """Config 5: Poisson arrivals, n = 2^k with k uniform in 12..20, b of 16..22
-    bits, 1000 squarings per request, admission by an HBM budget."""
+    bits taken from sieve survivors, 1000 squarings per request, admission by an
+    HBM budget."""
     import numpy as np
     rng = random.Random(SEED + 5)
+    pools = {}
     t_arr, reqs, t = [], [], 0.0
     for i in range(requests):
         t += rng.expovariate(rate)
         bits = rng.randrange(16, 23)
-        reqs.append((i, rng.randrange(12, 21), rng.randrange(1 << (bits - 1), 1 << bits), steps))
+        k = rng.randrange(12, 21)
+        reqs.append((i, k, rng.choice(sieve_pool(ct, k, bits, pools)), steps))
         t_arr.append(t)
     s = ct.Server(hbm_budget=budget, slots=16, chunk=125, max_steps=steps)
-    # warm-up: one short request per size class creates the resident class engines
-    # (tables, buffers) as a long-running server would already hold them
+    s.set_graphs(graphs)
+    # warm-up: one short request per size class creates the class engines
+    # (autotuned kind and tile, tables, buffers) as a long-running server would
+    # already hold them
     for k in range(12, 21):
         s.submit(10 ** 6 + k, k, 65537, 8)
     warm = 0
@@ -133,7 +153,8 @@ def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30):
             "wall_s": wall, "squarings_per_s": (st["squarings"] - sq0) / wall,
             "requests_per_s": requests / wall,
             "latency_p50_ms": 1e3 * float(np.percentile(L, 50)), "latency_p99_ms": 1e3 * float(np.percentile(L, 99)),
-            "hbm_budget_bytes": budget, "log2_n_range": [12, 20], "b_bits_range": [16, 22]}
+            "hbm_budget_bytes": budget, "log2_n_range": [12, 20], "b_bits_range": [16, 22],
+            "b_source": "GPU sieve survivors (both members, p < 2^26) of 2^16-wide windows"}
 
 
 def cpu_baseline(budget_s=12.0):
@@ -160,11 +181,14 @@ def cpu_baseline(budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=300)
-    ap.add_argument("--big-steps", type=int, default=200)
+    ap.add_argument("--big-steps", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="multipass engines launch every kernel instead of replaying captured HIP graphs "
+                         "(same results; used for kernel traces)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for hardware counters: 50 batched steps in one launch, 20 plain "
                          "squarings at n=2^14 and n=2^21, and 3 copies of an 8 MiB register "
@@ -179,8 +203,8 @@ def main():
     if args.profile:
         run_batched(ct, 50, 1)
         b0 = random.Random(SEED).randrange(1 << 18, 1 << 19)
-        run_plain(ct, 14, b0, 20, 1)
-        run_plain(ct, 21, 65536 + 2 * random.Random(SEED).randrange(1 << 14), 20, 1)
+        run_plain(ct, 14, b0, 20, 1, graphs=False)
+        run_plain(ct, 21, 65536 + 2 * random.Random(SEED).randrange(1 << 14), 20, 1, graphs=False)
         t = ct.Transform(21, 1, ct.KIND_PLAIN)
         for _ in range(3):
             t.copy(1, 0)
@@ -188,17 +212,17 @@ def main():
         print("[bench] profile run done (copy kernel moves 8388608 bytes each way)", flush=True)
         return
 
-    sq_s, ms, i0, bs = run_batched(ct, args.steps, args.warmup)
+    sq_s, ms, i0, bs, kind, tuning = run_batched(ct, args.steps, args.warmup)
     print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps)", flush=True)
-    plain_sq_s, plain_ms = run_plain(ct, 14, bs[0], args.plain_steps, 5)
+    plain_sq_s, plain_ms = run_plain(ct, 14, bs[0], args.plain_steps, 5, graphs=not args.no_graphs)
     print(f"[bench] plain n=2^14 x1: {plain_sq_s:.4g} squarings/s", flush=True)
     big_b = 65536 + 2 * random.Random(SEED).randrange(1 << 14)
-    big_sq_s, big_ms = run_plain(ct, 21, big_b, args.big_steps, 5)
+    big_sq_s, big_ms = run_plain(ct, 21, big_b, args.big_steps, 5, graphs=not args.no_graphs)
     print(f"[bench] plain n=2^21 x1: {big_sq_s:.4g} squarings/s", flush=True)
 
     sieve = run_sieve_config2(ct)
     print(f"[bench] sieve config 2: {sieve['survivors']} survivors in {sieve['seconds']:.3f} s", flush=True)
-    served = run_served(ct)
+    served = run_served(ct, graphs=not args.no_graphs)
     print(f"[bench] served: {served['squarings_per_s']:.4g} squarings/s, p50 {served['latency_p50_ms']:.1f} ms, "
           f"p99 {served['latency_p99_ms']:.1f} ms", flush=True)
 
@@ -222,7 +246,10 @@ def main():
         "data": "synthetic (seeded b)",
         "config": {"workload": "batched: 256 candidates b^(2^14) - b^(2^13) + 1, b in [2^18, 2^19), "
                                "base-2 PRP loop, one persistent resident launch",
-                   "log2_n": 14, "candidates": 256, "seed": SEED},
+                   "log2_n": 14, "candidates": 256, "seed": SEED,
+                   "engine": "resident" if kind == ct.KIND_RESIDENT else "multipass",
+                   "autotuner_ms_per_step": {"resident": tuning[1][0], "multipass_w16": tuning[1][1],
+                                             "multipass_w32": tuning[1][2]}},
         "candidates_per_hour": 3600.0 * sq_s / per_cand,
         "plain": {"squarings_per_s": plain_sq_s, "candidates_per_hour": 3600.0 * plain_sq_s / per_cand,
                   "engine": "plain multipass, one candidate per launch sequence"},

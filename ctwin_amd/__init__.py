@@ -32,7 +32,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
            "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
            "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
-           "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit"]
+           "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs"]
 
 INTERRUPTED = 1
 
@@ -96,6 +96,7 @@ def lib():
             "ct_server_stats": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u64), c.POINTER(u64)],
             "ct_engine_primes": [vp, c.POINTER(i32)],
             "ct_engine_moduli": [u32p],
+            "ct_server_set_graphs": [vp, i32],
             "ct_engine_set_graphs": [vp, i32],
             "ct_engine_set_column_tile": [vp, u32],
             "ct_sieve_save": [c.c_char_p, i32, u64, u64, u64, u64, u8p, i32],
@@ -285,6 +286,9 @@ class Server:
 
     def __del__(self):
         self.close()
+
+    def set_graphs(self, on):
+        _check(lib().ct_server_set_graphs(self.h, int(bool(on))))
 
     def submit(self, rid, log2_n, b, steps):
         _check(lib().ct_server_submit(self.h, int(rid), int(log2_n), int(b), int(steps)))

@@ -114,6 +114,7 @@ struct ct_server
 	std::map<uint32_t, std::unique_ptr<size_class>> classes;
 	std::vector<result> done;
 	uint64_t squarings = 0;
+	bool graphs = true;          // multipass classes replay captured graphs
 };
 
 static void destroy_class(ct_server *s, uint32_t ln)
@@ -164,6 +165,7 @@ static int make_class(ct_server *s, uint32_t ln, size_class **out)
 	int rc = ct_engine_create(s->device, ln, s->slots, 0, CT_KIND_AUTO, &c->e);
 	if (rc) return rc;
 	const uint64_t bytes = class_bytes(ln, s->slots, words, c->e->kind == CT_KIND_PLAIN);
+	c->e->use_graphs = s->graphs;
 	c->ln = ln; c->words = words; c->bytes = bytes;
 	c->lanes.resize(s->slots);
 	c->lane_i.assign(s->slots, -1);
@@ -315,6 +317,14 @@ int ct_server_poll(ct_server *s, uint64_t *ids, uint64_t *res64, uint64_t *hash,
 	}
 	s->done.erase(s->done.begin(), s->done.begin() + k);
 	*count = k;
+	return CT_OK;
+}
+
+int ct_server_set_graphs(ct_server *s, int on)
+{
+	if (!s) return fail(CT_ERR_ARG, "null server");
+	s->graphs = (on != 0);
+	for (auto &kv : s->classes) kv.second->e->use_graphs = s->graphs;
 	return CT_OK;
 }
 

@@ -169,6 +169,8 @@ int ct_server_step(ct_server *s);
 int ct_server_poll(ct_server *s, uint64_t *ids, uint64_t *res64, uint64_t *hash, double *latency_s, uint32_t max,
                    uint32_t *count);
 int ct_server_stats(const ct_server *s, uint32_t *queued, uint32_t *active, uint64_t *hbm_used, uint64_t *squarings);
+/* ct_engine_set_graphs for every size class of the server (default on). */
+int ct_server_set_graphs(ct_server *s, int on);
 
 /* ---------------- window sieve ---------------- */
 
