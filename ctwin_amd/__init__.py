@@ -32,7 +32,8 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
            "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
            "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
-           "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs"]
+           "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
+           "ct_engine_create_ex", "ct_engine_set_base"]
 
 INTERRUPTED = 1
 
@@ -96,6 +97,8 @@ def lib():
             "ct_server_stats": [vp, c.POINTER(u32), c.POINTER(u32), c.POINTER(u64), c.POINTER(u64)],
             "ct_engine_primes": [vp, c.POINTER(i32)],
             "ct_engine_moduli": [u32p],
+            "ct_engine_create_ex": [i32, u32, u32, u32, i32, i32, c.POINTER(vp)],
+            "ct_engine_set_base": [vp, u32],
             "ct_server_set_graphs": [vp, i32],
             "ct_engine_set_graphs": [vp, i32],
             "ct_engine_set_column_tile": [vp, u32],
@@ -127,14 +130,16 @@ def _check(rc):
 
 
 class Transform:
-    """transform + engine of cyclo/src: `lanes` candidates of n = 2^log2_n digits."""
+    """transform + engine of cyclo/src: `lanes` candidates of n = 2^log2_n digits,
+    modulo N+ = b^n - b^(n/2) + 1 (sign +1) or N- = b^n - b^(n/2) - 1 (sign -1)."""
 
-    def __init__(self, log2_n, lanes=1, kind=KIND_AUTO, device=0, csize=16):
+    def __init__(self, log2_n, lanes=1, kind=KIND_AUTO, device=0, csize=16, sign=1):
         self.n = 1 << log2_n
         self.log2_n = log2_n
         self.lanes = lanes
+        self.sign = sign
         h = ctypes.c_void_p()
-        _check(lib().ct_engine_create(device, log2_n, lanes, csize, kind, ctypes.byref(h)))
+        _check(lib().ct_engine_create_ex(device, log2_n, lanes, csize, kind, sign, ctypes.byref(h)))
         self.h = h
         n_ = ctypes.c_uint32(); l_ = ctypes.c_uint32(); k_ = ctypes.c_int()
         _check(lib().ct_engine_info(self.h, ctypes.byref(n_), ctypes.byref(l_), ctypes.byref(k_)))
@@ -234,6 +239,10 @@ class Transform:
         k, i, el = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _check(lib().ct_context_load(self.h, os.fsencode(path), ctypes.byref(k), ctypes.byref(i), ctypes.byref(el)))
         return k.value, i.value, el.value
+
+    def set_base(self, base):
+        """PRP base (2 or 3): a set exponent bit multiplies by it."""
+        _check(lib().ct_engine_set_base(self.h, int(base)))
 
     def set_graphs(self, on):
         """Multipass engine: replay captured HIP graphs for runs (default) or launch each kernel."""

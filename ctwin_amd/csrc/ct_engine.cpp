@@ -44,11 +44,13 @@ int ct::hip_fail(hipError_t e, const char *where)
 // ---------------------------------------------------------------------------
 // engine lifetime
 
-static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, uint32_t col_w, ct_engine **out)
+static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, uint32_t col_w, int sign, ct_engine **out)
 {
-	if (kind == CT_KIND_AUTO) kind = (log2_n <= 14) ? CT_KIND_RESIDENT : CT_KIND_PLAIN;
-	if (kind == CT_KIND_RESIDENT && log2_n > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need log2_n <= 14");
-	if (kind == CT_KIND_PLAIN && log2_n < 10) return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
+	const uint32_t tln = log2_n + (sign < 0 ? 1 : 0);
+	if (kind == CT_KIND_AUTO) kind = (tln <= 14) ? CT_KIND_RESIDENT : CT_KIND_PLAIN;
+	if (kind == CT_KIND_RESIDENT && tln > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need a transform length <= 2^14");
+	if (kind == CT_KIND_PLAIN && (tln < 10 || (sign < 0 && tln < 11)))
+		return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
 	if (kind != CT_KIND_RESIDENT && kind != CT_KIND_PLAIN) return fail(CT_ERR_ARG, "unknown engine kind");
 	int ndev = 0;
 	HIPCHK(hipGetDeviceCount(&ndev));
@@ -57,6 +59,7 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 
 	ct_engine *e = new ct_engine();
 	e->device = device; e->ln = log2_n; e->n = 1u << log2_n; e->lanes = lanes; e->csize = 16; e->kind = kind;
+	e->sign = (sign < 0) ? -1 : 1; e->tln = tln; e->tn = 1u << tln;
 	e->col_w = col_w;
 	e->gc = make_garner();
 	e->b.assign(lanes, 0);
@@ -68,7 +71,7 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 		if (h != hipSuccess) rc = fail(CT_ERR_NOMEM, "hipMalloc failed");
 	};
 	alloc((void **)&e->d_x, 3 * reg_words * sizeof(int32_t));
-	alloc((void **)&e->d_tw, 6 * (size_t)e->n * sizeof(tw_entry));
+	alloc((void **)&e->d_tw, 6 * (size_t)e->tn * sizeof(tw_entry));
 	alloc((void **)&e->d_dv, lanes * sizeof(divisor));
 	alloc((void **)&e->d_mask, ((lanes + 31) / 32) * sizeof(uint32_t));
 	if (rc == CT_OK && kind == CT_KIND_PLAIN) rc = plain_alloc(e);
@@ -77,7 +80,7 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 	if (h == hipSuccess) h = hipEventCreate(&e->ev0);
 	if (h == hipSuccess) h = hipEventCreate(&e->ev1);
 	if (h != hipSuccess) { ct_engine_destroy(e); return hip_fail(h, "stream/event create"); }
-	const std::vector<tw_entry> tw = build_tables(e->n);
+	const std::vector<tw_entry> tw = build_tables(e->tn);
 	h = hipMemcpy(e->d_tw, tw.data(), tw.size() * sizeof(tw_entry), hipMemcpyHostToDevice);
 	if (h == hipSuccess) h = hipMemset(e->d_x, 0, 3 * reg_words * sizeof(int32_t));
 	if (h != hipSuccess) { ct_engine_destroy(e); return hip_fail(h, "table upload"); }
@@ -96,9 +99,9 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 
 namespace {
 
-struct tune_key { int device; uint32_t ln, lanes; int kind;
+struct tune_key { int device; uint32_t ln, lanes; int kind, sign;
 	bool operator<(const tune_key &o) const
-	{ return std::tie(device, ln, lanes, kind) < std::tie(o.device, o.ln, o.lanes, o.kind); } };
+	{ return std::tie(device, ln, lanes, kind, sign) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign); } };
 struct tune_val { int kind; uint32_t col_w; float ms[3]; };
 std::mutex g_tune_mu;
 std::map<tune_key, tune_val> g_tune;
@@ -120,9 +123,10 @@ float time_squarings(ct_engine *e)
 
 }   // namespace
 
-static int tune(int device, uint32_t ln, uint32_t lanes, int kind, tune_val *out)
+static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tune_val *out)
 {
-	const tune_key key{device, ln, lanes, kind};
+	const tune_key key{device, ln, lanes, kind, sign};
+	const uint32_t tln = ln + (sign < 0 ? 1 : 0);
 	{
 		std::lock_guard<std::mutex> g(g_tune_mu);
 		auto it = g_tune.find(key);
@@ -130,12 +134,12 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, tune_val *out
 	}
 	tune_val v{CT_KIND_AUTO, 0, {-1, -1, -1}};
 	float best = 1e30f;
-	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && ln <= 14);
-	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && ln >= 10);
+	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && tln <= 14);
+	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && tln >= (sign < 0 ? 11u : 10u));
 	if (try_res)
 	{
 		ct_engine *e = nullptr;
-		int rc = create_impl(device, ln, lanes, CT_KIND_RESIDENT, 0, &e);
+		int rc = create_impl(device, ln, lanes, CT_KIND_RESIDENT, 0, sign, &e);
 		if (rc) return rc;
 		v.ms[0] = time_squarings(e);
 		ct_engine_destroy(e);
@@ -145,24 +149,24 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, tune_val *out
 	if (try_plain)
 	{
 		ct_engine *e = nullptr;
-		int rc = create_impl(device, ln, lanes, CT_KIND_PLAIN, 0, &e);
+		int rc = create_impl(device, ln, lanes, CT_KIND_PLAIN, 0, sign, &e);
 		if (rc) return rc;
 		const uint32_t tiles[2] = {16, 32};
 		for (int t = 0; t < 2; ++t)
 		{
-			if (!plain_tile_ok((int)ln, tiles[t])) continue;
+			if (!plain_tile_ok((int)tln, tiles[t])) continue;
 			e->col_w = tiles[t];
 			const float ms = time_squarings(e);
 			if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
 			v.ms[1 + t] = ms;
 			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; }
 		}
-		if (!plain_tile_ok((int)ln, 16) && !plain_tile_ok((int)ln, 32))
+		if (!plain_tile_ok((int)tln, 16) && !plain_tile_ok((int)tln, 32))
 		{   // small n: one tile only
 			const float ms = time_squarings(e);
 			if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
 			v.ms[1] = ms;
-			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = plain_default_tile((int)ln); }
+			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = plain_default_tile((int)tln); }
 		}
 		ct_engine_destroy(e);
 	}
@@ -175,30 +179,39 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, tune_val *out
 	return CT_OK;
 }
 
-int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out)
+int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, int sign, ct_engine **out)
 {
 	if (!out) return fail(CT_ERR_ARG, "out is null");
 	*out = nullptr;
-	if (log2_n < 5 || log2_n > (uint32_t)kMaxLog2N) return fail(CT_ERR_UNSUPPORTED, "log2_n must be in [5, 22]");
+	if (sign != 1 && sign != -1) return fail(CT_ERR_ARG, "sign must be +1 or -1");
+	const uint32_t tln = log2_n + (sign < 0 ? 1 : 0);
+	if (log2_n < 5 || tln > (uint32_t)kMaxLog2N)
+		return fail(CT_ERR_UNSUPPORTED, "log2_n must be in [5, 22] ([5, 21] for N-)");
 	if (lanes == 0) return fail(CT_ERR_ARG, "lanes must be > 0");
 	if (csize != 0 && csize != 16) return fail(CT_ERR_UNSUPPORTED, "csize must be 0 or 16");
 	if (kind != CT_KIND_AUTO && kind != CT_KIND_RESIDENT && kind != CT_KIND_PLAIN) return fail(CT_ERR_ARG, "unknown engine kind");
-	if (kind == CT_KIND_RESIDENT && log2_n > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need log2_n <= 14");
-	if (kind == CT_KIND_PLAIN && log2_n < 10) return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
-	if (csize == 16) return create_impl(device, log2_n, lanes, kind, 0, out);
+	if (kind == CT_KIND_RESIDENT && tln > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need a transform length <= 2^14");
+	if (kind == CT_KIND_PLAIN && (tln < 10 || (sign < 0 && tln < 11)))
+		return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
+	if (csize == 16) return create_impl(device, log2_n, lanes, kind, 0, sign, out);
 	tune_val v;
-	int rc = tune(device, log2_n, lanes, kind, &v);
+	int rc = tune(device, log2_n, lanes, kind, sign, &v);
 	if (rc) return rc;
-	rc = create_impl(device, log2_n, lanes, v.kind, v.col_w, out);
+	rc = create_impl(device, log2_n, lanes, v.kind, v.col_w, sign, out);
 	if (rc == CT_OK) std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
 	return rc;
+}
+
+int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out)
+{
+	return ct_engine_create_ex(device, log2_n, lanes, csize, kind, 1, out);
 }
 
 int ct_engine_set_column_tile(ct_engine *e, uint32_t w)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
 	if (e->kind != CT_KIND_PLAIN) return fail(CT_ERR_UNSUPPORTED, "column tiles belong to the multipass engine");
-	if (!plain_tile_ok((int)e->ln, w)) return fail(CT_ERR_UNSUPPORTED, "tile width not compiled for this length");
+	if (!plain_tile_ok((int)e->tln, w)) return fail(CT_ERR_UNSUPPORTED, "tile width not compiled for this length");
 	e->col_w = w;
 	return CT_OK;
 }
@@ -231,6 +244,14 @@ int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind)
 	if (n) *n = e->n;
 	if (lanes) *lanes = e->lanes;
 	if (kind) *kind = e->kind;
+	return CT_OK;
+}
+
+int ct_engine_set_base(ct_engine *e, uint32_t base)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (base != 2 && base != 3) return fail(CT_ERR_ARG, "the PRP base must be 2 or 3");
+	e->base = base;
 	return CT_OK;
 }
 
@@ -282,15 +303,18 @@ int ct_engine_set(ct_engine *e, uint32_t a)
 	return CT_OK;
 }
 
-// Every coefficient of a square (dup included) or product is bounded by
-// 3 n D^2, D >= |digit| (each product a_i a_k lands on at most two positions
-// of X^n - X^(n/2) + 1 and at most 1.5 n pairs meet at one position).  The
+// Every coefficient of a square or product is bounded by 3 n D^2 (N+) or
+// 4 n D^2 (N-), D >= |digit|.  N+: each product a_i a_k lands on at most two
+// positions of X^n - X^(n/2) + 1 and at most 1.5 n pairs meet at one position.
+// N-: the acyclic product has at most n D^2 per coefficient and the fold by
+// X^n = X^(n/2) + 1 adds at most four of them (2 c_{j+n} counts twice).  The
+// multiplication by the base happens after the CRT, on exact integers.  The
 // carry leaves |digit| <= b, so D = max(b, digits written or set).  Two primes
-// reconstruct exactly when 3 n D^2 < p1 p2 / 2.
+// reconstruct exactly when that bound is below p1 p2 / 2.
 void ct::choose_primes(ct_engine *e)
 {
 	const uint64_t D = std::max<uint64_t>(e->b_max, e->state_bound);
-	const unsigned __int128 bound = (unsigned __int128)3 * e->n * D * D;
+	const unsigned __int128 bound = (unsigned __int128)(e->sign < 0 ? 4 : 3) * e->n * D * D;
 	e->np = (e->force3 || bound >= e->gc.h2) ? 3 : 2;
 }
 
@@ -395,8 +419,8 @@ int ct::steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, 
 	res_args A;
 	A.x = e->reg(0); A.tw = e->d_tw; A.dv = e->d_dv;
 	A.bits = bits; A.bit_words = bit_words; A.mask = mask_words; A.lane_i = lane_i;
-	A.i_hi = i_hi; A.nsteps = nsteps; A.gc = e->gc;
-	HIPCHK(resident_steps((int)e->ln, e->np, A, e->lanes, e->st));
+	A.i_hi = i_hi; A.nsteps = nsteps; A.gc = e->gc; A.base_m1 = (int)e->base - 1;
+	HIPCHK(resident_steps((int)e->tln, e->np, e->sign < 0, A, e->lanes, e->st));
 	return CT_OK;
 }
 
@@ -405,7 +429,7 @@ int ct::mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
 	if (e->kind == CT_KIND_PLAIN) return plain_mul(e, dst, a, b);
 	mul_args A;
 	A.a = e->reg(a); A.b = e->reg(b); A.out = e->reg(dst); A.tw = e->d_tw; A.dv = e->d_dv; A.gc = e->gc;
-	HIPCHK(resident_mul((int)e->ln, e->np, A, e->lanes, e->st));
+	HIPCHK(resident_mul((int)e->tln, e->np, e->sign < 0, A, e->lanes, e->st));
 	return CT_OK;
 }
 
@@ -475,8 +499,49 @@ int ct_engine_run(ct_engine *e, int i_hi, int i_lo, int gl_block)
 // Canonical digits of one lane, the algorithm of transform.h:501-597:
 // floor-carry in base b, wrap X^n = X^(n/2) - 1 until no carry is left, then
 // subtract N once when the value is >= N.
-int ct::canonical(const int32_t *d, uint32_t n, uint32_t b, uint32_t *x)
+// N- = b^n - b^(n/2) - 1 (no counterpart in the reference): carry with
+// floor division, wrap the top carry f as f (b^(n/2) + 1), then subtract N-
+// once if the value reached it (digits of N-: b - 1 everywhere but b - 2 at n/2).
+static int canonical_minus(const int32_t *d, uint32_t n, uint32_t b, uint32_t *x)
 {
+	auto fl = [b](int64_t &f) -> uint32_t {
+		int64_t q = f / (int64_t)b, r = f % (int64_t)b;
+		if (r < 0) { r += b; --q; }
+		f = q;
+		return (uint32_t)r;
+	};
+	int64_t f = 0;
+	for (uint32_t j = 0; j < n; ++j) { int64_t v = f + d[j]; x[j] = fl(v); f = v; }
+	int guard = 0;
+	while (f != 0)
+	{
+		if (++guard > 64) return CT_ERR_GETINT;
+		int64_t v = f;
+		for (uint32_t j = 0; j < n / 2; ++j) { v += (int64_t)x[j]; x[j] = fl(v); }
+		v += f;
+		for (uint32_t j = n / 2; j < n; ++j) { v += (int64_t)x[j]; x[j] = fl(v); }
+		f = v;
+	}
+	auto nd = [&](uint32_t j) -> uint32_t { return (j == n / 2) ? b - 2 : b - 1; };
+	int cmp = 0;
+	for (uint32_t j = n; j-- > 0;) if (x[j] != nd(j)) { cmp = (x[j] > nd(j)) ? 1 : -1; break; }
+	if (cmp >= 0)
+	{   // value - N-
+		int64_t borrow = 0;
+		for (uint32_t j = 0; j < n; ++j)
+		{
+			int64_t v = (int64_t)x[j] - nd(j) - borrow;
+			borrow = (v < 0) ? 1 : 0;
+			x[j] = (uint32_t)(v + (borrow ? (int64_t)b : 0));
+		}
+		if (borrow) return CT_ERR_GETINT;
+	}
+	return CT_OK;
+}
+
+int ct::canonical(const int32_t *d, uint32_t n, uint32_t b, int sign, uint32_t *x)
+{
+	if (sign < 0) return canonical_minus(d, n, b, x);
 	auto fl = [b](int64_t &f) -> uint32_t {
 		int64_t q = f / (int64_t)b, r = f % (int64_t)b;
 		if (r < 0) { r += b; --q; }
@@ -526,7 +591,7 @@ int ct_engine_get_int(ct_engine *e, uint32_t reg, uint32_t *digits)
 	if (rc) return rc;
 	for (uint32_t l = 0; l < e->lanes; ++l)
 	{
-		rc = canonical(&raw[(size_t)l * e->n], e->n, e->b[l], &digits[(size_t)l * e->n]);
+		rc = canonical(&raw[(size_t)l * e->n], e->n, e->b[l], e->sign, &digits[(size_t)l * e->n]);
 		if (rc) return fail(CT_ERR_GETINT, "getInt failed");
 	}
 	return CT_OK;

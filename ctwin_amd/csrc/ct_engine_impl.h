@@ -12,7 +12,9 @@
 struct ct_engine
 {
 	int device = 0;
-	uint32_t ln = 0, n = 0, lanes = 0, csize = 16;
+	uint32_t ln = 0, n = 0, lanes = 0, csize = 16;   // n = 2^ln digits per candidate
+	int sign = 1;                     // N = b^n - b^(n/2) + sign
+	uint32_t tln = 0, tn = 0;         // transform length 2^tln: n (N+) or 2n (N-, exact product then fold)
 	int kind = 0;
 	hipStream_t st = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -37,14 +39,17 @@ struct ct_engine
 	uint64_t b_max = 0;               // largest b over the lanes
 	uint64_t state_bound = 1;         // bound on |digit| set or written into the registers
 	bool force3 = false;              // always use three primes
+	uint32_t base = 2;                // PRP base: a set exponent bit multiplies by it (2 or 3)
 	// captured squaring sequences of the multipass engine (hipGraph), keyed
 	// by what the captured kernel arguments depend on
-	struct graph_entry { int nsteps, np; uint32_t col_w; bool first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
+	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
 	std::vector<graph_entry> graphs;
 	bool use_graphs = true;
 	int32_t *d_run = nullptr;         // {top bit index of the run, squarings done by replayed graphs}
 
 	int32_t *reg(uint32_t r) const { return d_x + (size_t)r * lanes * n; }
+	// multipass: rows of the R x C transform layout that hold digits
+	uint32_t digit_rows() const { return (1u << rows_log2) >> (sign < 0 ? 1 : 0); }
 	size_t reg_bytes() const { return (size_t)lanes * n * sizeof(int32_t); }
 };
 
@@ -52,7 +57,7 @@ namespace ct {
 
 int fail(int code, const char *what);
 int hip_fail(hipError_t e, const char *where);
-int canonical(const int32_t *d, uint32_t n, uint32_t b, uint32_t *x);
+int canonical(const int32_t *d, uint32_t n, uint32_t b, int sign, uint32_t *x);
 int steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, int nsteps, const uint32_t *mask_words,
           const int32_t *lane_i = nullptr);
 int mul_regs(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);

@@ -9,7 +9,7 @@
 
 namespace ct {
 
-// k_res_steps: x <- x^2 * 2^bit for nsteps steps, bit i = i_hi, i_hi - 1, ...
+// k_res_steps: x <- x^2 * base^bit for nsteps steps, bit i = i_hi, i_hi - 1, ...
 struct res_args
 {
 	int32_t *x;                 // [lanes][n] signed digits (register 0)
@@ -22,6 +22,7 @@ struct res_args
 	int nsteps;
 	const int32_t *lane_i;      // per-lane top bit index (continuous batching), or null: i_hi;
 	                            // a negative entry marks an idle lane (its workgroup exits)
+	int base_m1;                // PRP base - 1 (a set bit multiplies by the base)
 	garner_consts gc;
 };
 
@@ -35,7 +36,7 @@ struct mul_args
 	garner_consts gc;
 };
 
-hipError_t resident_steps(int ln, int np, const res_args &A, uint32_t lanes, hipStream_t st);
-hipError_t resident_mul(int ln, int np, const mul_args &A, uint32_t lanes, hipStream_t st);
+hipError_t resident_steps(int tln, int np, bool minus, const res_args &A, uint32_t lanes, hipStream_t st);
+hipError_t resident_mul(int tln, int np, bool minus, const mul_args &A, uint32_t lanes, hipStream_t st);
 
 }   // namespace ct

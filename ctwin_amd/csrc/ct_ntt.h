@@ -432,14 +432,17 @@ __device__ __forceinline__ int64_t garner2(uint32_t r1, uint32_t r2, const garne
 }
 
 // normalize1 with a 64-bit accumulator (two-prime mode: |l| < 2^61)
-__device__ __forceinline__ int64_t carry_chunk64(const int64_t (&l)[E], int32_t (&dig)[E], bool dup, const divisor &dv)
+// dup: multiply the chunk by the PRP base (2 or 3; base_m1 = base - 1) as it is
+// carried, x^2 * base^bit (kernel.cl:1044 doubles; base 3 adds l twice).
+__device__ __forceinline__ int64_t carry_chunk64(const int64_t (&l)[E], int32_t (&dig)[E], bool dup, int base_m1,
+                                                 const divisor &dv)
 {
 	int64_t acc = 0;
 #pragma unroll
 	for (int q = 0; q < E; ++q)
 	{
 		acc += l[q];
-		if (dup) acc += l[q];
+		if (dup) { acc += l[q]; if (base_m1 == 2) acc += l[q]; }
 		dig[q] = divrem_i64(acc, dv);
 	}
 	return acc;
@@ -447,14 +450,15 @@ __device__ __forceinline__ int64_t carry_chunk64(const int64_t (&l)[E], int32_t 
 
 // normalize1 on one 16-coefficient chunk (kernel.cl:1029-1051): returns the
 // chunk's outgoing carry.
-__device__ __forceinline__ int64_t carry_chunk(const i96 (&l)[E], int32_t (&dig)[E], bool dup, const divisor &dv)
+__device__ __forceinline__ int64_t carry_chunk(const i96 (&l)[E], int32_t (&dig)[E], bool dup, int base_m1,
+                                               const divisor &dv)
 {
 	i96 acc; acc.lo = 0; acc.hi = 0;
 #pragma unroll
 	for (int q = 0; q < E; ++q)
 	{
 		acc = i96_add(acc, l[q]);
-		if (dup) acc = i96_add(acc, l[q]);
+		if (dup) { acc = i96_add(acc, l[q]); if (base_m1 == 2) acc = i96_add(acc, l[q]); }
 		dig[q] = divrem_i96(acc, dv);
 	}
 	return (int64_t)acc.lo;

@@ -43,6 +43,7 @@ struct plain_args
 	const int32_t *run;         // device {i0, iter}: top bit index of the run, squarings done by earlier graph replays
 	const int32_t *lane_i;      // per-lane top bit index (or null, then i0); negative = idle lane
 	int step;                   // squaring index within this launch sequence: bit = top - (iter + step)
+	int base_m1;                // PRP base - 1 (a set bit multiplies by the base)
 	uint32_t lanes;
 	garner_consts gc;
 };
@@ -125,17 +126,20 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 	}
 }
 
-// A: chunk load (+ pending carry), column levels, residue planes
-template <int LN, int RL, int NP, int W>
+// A: chunk load (+ pending carry), column levels, residue planes.  MINUS: the
+// transform has length 2n (n = nd digits in rows < R/2, zeros below), for the
+// exact product modulo N- = b^nd - b^(nd/2) - 1 (folded in pass C).
+template <int LN, int RL, int NP, int W, bool MINUS>
 __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_args A, int operand)
 {
 	using G = colgeo<LN, RL, W>;
-	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
+	constexpr uint32_t n = 1u << LN, C = n >> RL;
+	constexpr uint32_t nd = MINUS ? n / 2 : n, nc = nd / 16, RD = MINUS ? G::R / 2 : G::R;   // digits, chunks, digit rows
 	__shared__ int32_t D[G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
 	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
 	if (A.lane_i && A.lane_i[lane] < 0) return;
-	const int32_t *x = (operand ? A.xb : A.x) + (size_t)lane * n;
+	const int32_t *x = (operand ? A.xb : A.x) + (size_t)lane * nd;
 	uint32_t *res = (operand ? A.res2 : A.res) + (size_t)lane * n;
 	// chunk role: row r, chunk h of the W columns
 	{
@@ -143,17 +147,24 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_arg
 		const uint32_t r = t / hpr, h = t % hpr;
 		const uint32_t k = r * C + c0 + 16 * h;
 		int32_t dig[E];
-		const int4 *src = reinterpret_cast<const int4 *>(x + k);
 #pragma unroll
-		for (int v = 0; v < 4; ++v) { const int4 w = src[v]; dig[4 * v] = w.x; dig[4 * v + 1] = w.y; dig[4 * v + 2] = w.z; dig[4 * v + 3] = w.w; }
-		if (A.carry_in && operand == 0)
+		for (int q = 0; q < E; ++q) dig[q] = 0;
+		if (r < RD)
 		{
-			const uint32_t g = k / 16;
-			const int64_t *f = A.carry_in + (size_t)lane * nc;
-			int64_t c = f[cslot((g + nc - 1) % nc, C / 16, G::R)];
-			if (g == 0) c = -c;
-			else if (g == nc / 2) c += f[cslot(nc - 1, C / 16, G::R)];
-			carry_in(dig, c, A.dv[lane]);
+			const int4 *src = reinterpret_cast<const int4 *>(x + k);
+#pragma unroll
+			for (int v = 0; v < 4; ++v) { const int4 w = src[v]; dig[4 * v] = w.x; dig[4 * v + 1] = w.y; dig[4 * v + 2] = w.z; dig[4 * v + 3] = w.w; }
+			if (A.carry_in && operand == 0)
+			{
+				// X^nd = X^(nd/2) -+ 1: the top carry goes to chunk 0 (negated for
+				// N+) and is added to chunk nc/2
+				const uint32_t g = k / 16;
+				const int64_t *f = A.carry_in + (size_t)lane * nc;
+				int64_t c = f[cslot((g + nc - 1) % nc, C / 16, RD)];
+				if (g == 0) c = MINUS ? c : -c;
+				else if (g == nc / 2) c += f[cslot(nc - 1, C / 16, RD)];
+				carry_in(dig, c, A.dv[lane]);
+			}
 		}
 #pragma unroll
 		for (int q = 0; q < E; ++q) D[(16 * h + q) * G::RP + r + (r >> 5)] = dig[q];
@@ -207,12 +218,26 @@ __global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
 	else rowB_prime<LN, RL, 2, MUL>(A, lds, r, lane, t);
 }
 
-// C: inverse column levels, CRT, normalize1
-template <int LN, int RL, int NP, int W>
+// C: inverse column levels, CRT, normalize1.  MINUS: the 2n coefficients of
+// each column are folded by X^nd = X^(nd/2) + 1 in residue space first
+// (rows r < R/4: c_r + c_{r+R/2} + c_{r+3R/4}; R/4 <= r < R/2: c_r + c_{r+R/4}
+// + 2 c_{r+R/2}), leaving the nd = n/2 digit coefficients in rows < R/2.
+template <int P>
+__device__ __forceinline__ uint32_t fold_minus(const uint32_t *S, uint32_t col_base, uint32_t r, uint32_t R)
+{
+	constexpr uint32_t p = kP[P];
+	auto at = [&](uint32_t row) { return S[col_base + row + (row >> 5)]; };
+	if (r < R / 4) return red1p(red1p(at(r) + at(r + R / 2), p) + at(r + 3 * R / 4), p);
+	const uint32_t c2 = at(r + R / 4 + R / 4);
+	return red1p(red1p(at(r) + at(r + R / 4), p) + red1p(c2 + c2, p), p);
+}
+
+template <int LN, int RL, int NP, int W, bool MINUS>
 __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_args A, int no_dup)
 {
 	using G = colgeo<LN, RL, W>;
-	constexpr uint32_t n = 1u << LN, C = n >> RL, nc = n / 16;
+	constexpr uint32_t n = 1u << LN, C = n >> RL;
+	constexpr uint32_t nd = MINUS ? n / 2 : n, nc = nd / 16, RD = MINUS ? G::R / 2 : G::R;
 	constexpr int La = G::passes - 1;
 	__shared__ uint32_t S[3][G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
@@ -246,9 +271,10 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 		for (int q = 0; q < E; ++q) sw[loff<RL, 0>(q)] = uu[P][q];
 	}
 	__syncthreads();
-	// chunk role
+	// chunk role (threads whose row holds digits)
 	constexpr uint32_t hpr = G::W / 16;
 	const uint32_t r = t / hpr, h = t % hpr;
+	if (r >= RD) return;
 	const uint32_t k = r * C + c0 + 16 * h;
 	const uint32_t rr = r + (r >> 5);
 	bool dup = false;
@@ -266,10 +292,13 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 #pragma unroll
 		for (int q = 0; q < E; ++q)
 		{
-			const uint32_t a = (16 * h + q) * G::RP + rr;
-			l[q] = garner(S[0][a], S[1][a], S[2][a], A.gc);
+			const uint32_t cb = (16 * h + q) * G::RP, a = cb + rr;
+			if constexpr (MINUS)
+				l[q] = garner(fold_minus<0>(S[0], cb, r, G::R), fold_minus<1>(S[1], cb, r, G::R),
+				              fold_minus<2>(S[2], cb, r, G::R), A.gc);
+			else l[q] = garner(S[0][a], S[1][a], S[2][a], A.gc);
 		}
-		f = carry_chunk(l, dig, dup, A.dv[lane]);
+		f = carry_chunk(l, dig, dup, A.base_m1, A.dv[lane]);
 	}
 	else
 	{
@@ -277,19 +306,22 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 #pragma unroll
 		for (int q = 0; q < E; ++q)
 		{
-			const uint32_t a = (16 * h + q) * G::RP + rr;
-			l[q] = garner2(S[0][a], S[1][a], A.gc);
+			const uint32_t cb = (16 * h + q) * G::RP, a = cb + rr;
+			if constexpr (MINUS) l[q] = garner2(fold_minus<0>(S[0], cb, r, G::R), fold_minus<1>(S[1], cb, r, G::R), A.gc);
+			else l[q] = garner2(S[0][a], S[1][a], A.gc);
 		}
-		f = carry_chunk64(l, dig, dup, A.dv[lane]);
+		f = carry_chunk64(l, dig, dup, A.base_m1, A.dv[lane]);
 	}
-	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * n + k);
+	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * nd + k);
 #pragma unroll
 	for (int v = 0; v < 4; ++v) dst[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
-	A.carry[(size_t)lane * nc + cslot(k / 16, C / 16, G::R)] = f;
+	A.carry[(size_t)lane * nc + cslot(k / 16, C / 16, RD)] = f;
 }
 
 // normalize2 alone: materialises the pending carries into the digits
-__global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, uint32_t n, uint32_t lanes, uint32_t rows)
+// n = digits per lane, rows = rows holding digits, minus: N- wrap
+__global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, uint32_t n, uint32_t lanes, uint32_t rows,
+                        int minus)
 {
 	const uint32_t nc = n / 16, cpr = nc / rows;
 	const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,7 +329,7 @@ __global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, ui
 	const uint32_t lane = (uint32_t)(id / nc), g = (uint32_t)(id % nc);
 	const int64_t *f = carry + (size_t)lane * nc;
 	int64_t c = f[cslot((g + nc - 1) % nc, cpr, rows)];
-	if (g == 0) c = -c;
+	if (g == 0) c = minus ? c : -c;
 	else if (g == nc / 2) c += f[cslot(nc - 1, cpr, rows)];
 	int4 *p = reinterpret_cast<int4 *>(x + (size_t)lane * n + 16 * (size_t)g);
 	int32_t dig[E];
@@ -323,77 +355,95 @@ static int rows_log2(int ln) { return (ln <= 12) ? 4 : 8; }
 static bool tile_ok(int ln, uint32_t w) { return (ln <= 12) ? (w == 64) : (w == 16 || w == 32); }
 uint32_t plain_default_tile(int ln) { return (ln <= 12) ? 64 : ((ln >= 19) ? 32 : 16); }
 
-template <int LN, int RL, int NP, int W>
+template <int LN, int RL, int NP, int W, bool MINUS>
 static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
 {
 	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, A, 0);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
 	hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
-	hipLaunchKernelGGL((k_colC<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, A, 0);
+	hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
 	return hipGetLastError();
 }
 
-template <int LN, int RL, int NP, int W>
+template <int LN, int RL, int NP, int W, bool MINUS>
 static hipError_t mul_ln(const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
 {
 	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, In, 0);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, In, 1);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, In, 0);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, In, 1);
 	hipLaunchKernelGGL((k_rowB<LN, RL, true>), dim3(R, NP, lanes), dim3(C / 16), 0, st, Out);
-	hipLaunchKernelGGL((k_colC<LN, RL, NP, W>), gcol, dim3(G::threads), 0, st, Out, 1);
+	hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, Out, 1);
 	return hipGetLastError();
 }
 
-template <int LN, int RL, int W>
+template <int LN, int RL, int W, bool MINUS>
 static hipError_t square_np(int np, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	return (np == 2) ? square_ln<LN, RL, 2, W>(A, lanes, st) : square_ln<LN, RL, 3, W>(A, lanes, st);
+	return (np == 2) ? square_ln<LN, RL, 2, W, MINUS>(A, lanes, st) : square_ln<LN, RL, 3, W, MINUS>(A, lanes, st);
 }
 
-template <int LN, int RL, int W>
+template <int LN, int RL, int W, bool MINUS>
 static hipError_t mul_np(int np, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
 {
-	return (np == 2) ? mul_ln<LN, RL, 2, W>(In, Out, lanes, st) : mul_ln<LN, RL, 3, W>(In, Out, lanes, st);
+	return (np == 2) ? mul_ln<LN, RL, 2, W, MINUS>(In, Out, lanes, st) : mul_ln<LN, RL, 3, W, MINUS>(In, Out, lanes, st);
 }
 
-template <int LN, int RL>
+template <int LN, int RL, bool MINUS>
 static hipError_t square_w(int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	if constexpr (RL == 4) return square_np<LN, RL, 64>(np, A, lanes, st);
-	else return (w == 32) ? square_np<LN, RL, 32>(np, A, lanes, st) : square_np<LN, RL, 16>(np, A, lanes, st);
+	if constexpr (RL == 4) return square_np<LN, RL, 64, MINUS>(np, A, lanes, st);
+	else return (w == 32) ? square_np<LN, RL, 32, MINUS>(np, A, lanes, st) : square_np<LN, RL, 16, MINUS>(np, A, lanes, st);
+}
+
+template <int LN, int RL, bool MINUS>
+static hipError_t mul_w(int np, uint32_t w, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+{
+	if constexpr (RL == 4) return mul_np<LN, RL, 64, MINUS>(np, In, Out, lanes, st);
+	else return (w == 32) ? mul_np<LN, RL, 32, MINUS>(np, In, Out, lanes, st) : mul_np<LN, RL, 16, MINUS>(np, In, Out, lanes, st);
 }
 
 template <int LN, int RL>
-static hipError_t mul_w(int np, uint32_t w, const plain_args &In, const plain_args &Out, uint32_t lanes, hipStream_t st)
+static hipError_t square_m(int np, uint32_t w, bool minus, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	if constexpr (RL == 4) return mul_np<LN, RL, 64>(np, In, Out, lanes, st);
-	else return (w == 32) ? mul_np<LN, RL, 32>(np, In, Out, lanes, st) : mul_np<LN, RL, 16>(np, In, Out, lanes, st);
+	if constexpr (LN >= 11)
+		if (minus) return square_w<LN, RL, true>(np, w, A, lanes, st);
+	return minus ? hipErrorInvalidValue : square_w<LN, RL, false>(np, w, A, lanes, st);
+}
+
+template <int LN, int RL>
+static hipError_t mul_m(int np, uint32_t w, bool minus, const plain_args &In, const plain_args &Out, uint32_t lanes,
+                        hipStream_t st)
+{
+	if constexpr (LN >= 11)
+		if (minus) return mul_w<LN, RL, true>(np, w, In, Out, lanes, st);
+	return minus ? hipErrorInvalidValue : mul_w<LN, RL, false>(np, w, In, Out, lanes, st);
 }
 
 #define CT_PLAIN_SIZES(X) X(10, 4) X(11, 4) X(12, 4) X(13, 8) X(14, 8) X(15, 8) X(16, 8) X(17, 8) \
 	X(18, 8) X(19, 8) X(20, 8) X(21, 8) X(22, 8)
 
-static hipError_t square_any(int ln, int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st)
+// tln: log2 of the transform length (n digits for N+, 2n for N-)
+static hipError_t square_any(int tln, int np, uint32_t w, bool minus, const plain_args &A, uint32_t lanes, hipStream_t st)
 {
-	switch (ln)
+	switch (tln)
 	{
-#define CT_SQ_CASE(L_, R_) case L_: return square_w<L_, R_>(np, w, A, lanes, st);
+#define CT_SQ_CASE(L_, R_) case L_: return square_m<L_, R_>(np, w, minus, A, lanes, st);
 	CT_PLAIN_SIZES(CT_SQ_CASE)
 #undef CT_SQ_CASE
 	default: return hipErrorInvalidValue;
 	}
 }
 
-static hipError_t mul_any(int ln, int np, uint32_t w, const plain_args &In, const plain_args &Out, uint32_t lanes,
-                          hipStream_t st)
+static hipError_t mul_any(int tln, int np, uint32_t w, bool minus, const plain_args &In, const plain_args &Out,
+                          uint32_t lanes, hipStream_t st)
 {
-	switch (ln)
+	switch (tln)
 	{
-#define CT_MUL_CASE(L_, R_) case L_: return mul_w<L_, R_>(np, w, In, Out, lanes, st);
+#define CT_MUL_CASE(L_, R_) case L_: return mul_m<L_, R_>(np, w, minus, In, Out, lanes, st);
 	CT_PLAIN_SIZES(CT_MUL_CASE)
 #undef CT_MUL_CASE
 	default: return hipErrorInvalidValue;
@@ -406,10 +456,11 @@ bool plain_tile_ok(int ln, uint32_t w) { return tile_ok(ln, w); }
 
 int plain_alloc(ct_engine *e)
 {
-	if (e->ln < 10 || e->ln > 22) return fail(CT_ERR_UNSUPPORTED, "plain engine needs 10 <= log2_n <= 22");
-	e->rows_log2 = (uint32_t)rows_log2((int)e->ln);
-	if (!tile_ok((int)e->ln, e->col_w)) e->col_w = plain_default_tile((int)e->ln);
-	const size_t words = (size_t)3 * e->lanes * e->n;
+	if (e->tln < 10 || e->tln > 22 || (e->sign < 0 && e->tln < 11))
+		return fail(CT_ERR_UNSUPPORTED, "multipass engine needs a transform length 2^10 .. 2^22 (2^11 .. for N-)");
+	e->rows_log2 = (uint32_t)rows_log2((int)e->tln);
+	if (!tile_ok((int)e->tln, e->col_w)) e->col_w = plain_default_tile((int)e->tln);
+	const size_t words = (size_t)3 * e->lanes * e->tn;
 	if (hipMalloc((void **)&e->d_res, words * sizeof(uint32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc residues");
 	if (hipMalloc((void **)&e->d_res2, words * sizeof(uint32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc residues");
 	if (hipMalloc((void **)&e->d_carry, (size_t)e->lanes * (e->n / 16) * sizeof(int64_t)) != hipSuccess)
@@ -436,7 +487,7 @@ static plain_args make_args(ct_engine *e)
 	plain_args A;
 	std::memset(&A, 0, sizeof(A));
 	A.res = e->d_res; A.res2 = e->d_res2; A.carry = e->d_carry;
-	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc; A.run = e->d_run;
+	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc; A.run = e->d_run; A.base_m1 = (int)e->base - 1;
 	return A;
 }
 
@@ -446,7 +497,7 @@ int plain_materialize(ct_engine *e)
 	if (!e->pending) return CT_OK;
 	const size_t total = (size_t)e->lanes * (e->n / 16);
 	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,
-	                   e->reg(0), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes, 1u << e->rows_log2);
+	                   e->reg(0), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes, e->digit_rows(), e->sign < 0 ? 1 : 0);
 	HIPCHK(hipGetLastError());
 	e->pending = false;
 	return CT_OK;
@@ -462,7 +513,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 {
 	const bool fc = e->pending;
 	for (const auto &g : e->graphs)
-		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
+		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.base == e->base && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
 		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
 	if (e->graphs.size() >= kGraphCache) drop_graphs(e);
 	hipGraph_t graph = nullptr;
@@ -472,7 +523,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 	{
 		A.step = j;
 		A.carry_in = (j == 0 && !fc) ? nullptr : e->d_carry;
-		h = square_any((int)e->ln, e->np, e->col_w, A, e->lanes, e->st);
+		h = square_any((int)e->tln, e->np, e->col_w, e->sign < 0, A, e->lanes, e->st);
 	}
 	if (h == hipSuccess)
 	{
@@ -489,7 +540,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	hipGraphDestroy(graph);
 	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
-	e->graphs.push_back({len, e->np, e->col_w, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
+	e->graphs.push_back({len, e->np, e->col_w, e->base, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
 	*out = exec;
 	return CT_OK;
 }
@@ -520,7 +571,7 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 	{
 		A.step = s - in_graphs;
 		A.carry_in = e->pending ? e->d_carry : nullptr;
-		HIPCHK(square_any((int)e->ln, e->np, e->col_w, A, e->lanes, e->st));
+		HIPCHK(square_any((int)e->tln, e->np, e->col_w, e->sign < 0, A, e->lanes, e->st));
 		e->pending = true;
 	}
 	return CT_OK;
@@ -537,11 +588,11 @@ int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b)
 	In.x = e->reg(a); In.xb = e->reg(b);
 	plain_args Out = make_args(e);
 	Out.x = e->reg(dst);
-	HIPCHK(mul_any((int)e->ln, e->np, e->col_w, In, Out, e->lanes, e->st));
+	HIPCHK(mul_any((int)e->tln, e->np, e->col_w, e->sign < 0, In, Out, e->lanes, e->st));
 	// materialise dst at once (only register 0 may carry pending carries)
 	const size_t total = (size_t)e->lanes * (e->n / 16);
 	hipLaunchKernelGGL(k_carry, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, e->st,
-	                   e->reg(dst), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes, 1u << e->rows_log2);
+	                   e->reg(dst), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes, e->digit_rows(), e->sign < 0 ? 1 : 0);
 	HIPCHK(hipGetLastError());
 	return CT_OK;
 }

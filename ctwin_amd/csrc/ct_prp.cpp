@@ -1,6 +1,7 @@
-// Base-2 Fermat PRP driver with Gerbicz-Li verification: cyclo::check of
-// cyclo/src/cyclo.h:252-411 over the C ABI of the engine.  The exponent
-// E = b^n - b^(n/2) is built with GMP as in the reference (cyclo.h:258-271),
+// Fermat PRP driver with Gerbicz-Li verification: cyclo::check of
+// cyclo/src/cyclo.h:252-411 over the C ABI of the engine (base 2 as in the
+// reference, or the engine's base 3; N+ as in the reference, or N- for an N-
+// engine).  The exponent E = N - 1 is built with GMP as in the reference (cyclo.h:258-271),
 // but its bits are uploaded once and read by the kernels instead of being
 // gathered on the host every iteration (cyclo.h:244-249).
 #include <algorithm>
@@ -43,7 +44,8 @@ int ct::prp_begin(ct_engine *e, const uint32_t *b, prp_job &J)
 	const uint32_t lanes = e->lanes, n = e->n;
 	for (auto &z : J.ex) mpz_clear(&z);
 	J.ex.assign(lanes, __mpz_struct());
-	// E = (b^(n/2))^2 - b^(n/2); i0 = top bit over all lanes (cyclo.h:256-272)
+	// E = N - 1 = (b^(n/2))^2 - b^(n/2) (N+; minus 2 more for N-); i0 = top bit
+	// over all lanes (cyclo.h:256-272)
 	J.i0 = 0;
 	mpz_t h; mpz_init(h);
 	for (uint32_t l = 0; l < lanes; ++l)
@@ -52,6 +54,7 @@ int ct::prp_begin(ct_engine *e, const uint32_t *b, prp_job &J)
 		mpz_ui_pow_ui(h, b[l], n / 2);
 		mpz_mul(&J.ex[l], h, h);
 		mpz_sub(&J.ex[l], &J.ex[l], h);
+		if (e->sign < 0) mpz_sub_ui(&J.ex[l], &J.ex[l], 2);     // N- - 1 = b^n - b^(n/2) - 2
 		J.i0 = std::max(J.i0, (int)mpz_sizeinbase(&J.ex[l], 2) - 1);
 	}
 	mpz_clear(h);

@@ -18,6 +18,16 @@
 //   L == a     : k = 16 t + q  (final pass, and the carry chunks)
 // Layout 0 (k = t + q*T) is also the layout of the digits in registers and in
 // HBM between launches, so loads and stores are coalesced.
+//
+// MINUS (N- = b^n - b^(n/2) - 1, n = 2^(LN-1) digits): the transform has length
+// 2n; the digits fill k < n (slots q < 8 of layout 0) and the rest stays zero,
+// so the transform's product is the exact acyclic product c_0 .. c_{2n-2}.  It
+// is folded by X^n = X^(n/2) + 1 in residue space inside each thread:
+//   j < n/2:      d_j = c_j + c_{j+n} + c_{j+3n/2}
+//   n/2 <= j < n: d_j = c_j + c_{j+n/2} + 2 c_{j+n}
+// (k + n/2, k + n, k + 3n/2 are slots q + 4, q + 8, q + 12), then the CRT and
+// the carry run on the n digits (threads t < T/2), the top carry wrapping to
+// chunk 0 and chunk n/32 with a plus sign.
 #include <hip/hip_runtime.h>
 
 #include "ct_arith.h"
@@ -27,10 +37,11 @@
 
 namespace ct {
 
-// x <- x^2 (or x*y) mod (X^n - X^(n/2) + 1, p_P); the signed digits are read
+// x <- x^2 (or x*y) mod (X^n - X^(n/2) + 1, p_P) -- or, MINUS, the folded
+// product mod X^(n/2) - X^(n/4) - 1 in slots q < 8; the signed digits are read
 // from LDS (layout 0, region D), the output residues (exact coefficients mod
-// p) are left in registers in layout 0.
-template <int LN, int P, bool MUL>
+// p, in [0, p)) are left in registers in layout 0.
+template <int LN, int P, bool MUL, bool MINUS>
 __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[E], uint32_t (&out)[E],
                                           uint32_t *lds, const tw_entry *tw, uint32_t t)
 {
@@ -54,65 +65,87 @@ __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[
 		for (int k = 0; k < E; ++k) v[k] = 0;
 	}
 	final_inv<LN, P, true, MUL, false>(u, v, lds, twf, twi, 1u, t, F);
+	if constexpr (MINUS)
+	{
 #pragma unroll
-	for (int k = 0; k < E; ++k) out[k] = u[k];
+		for (int q = 0; q < 4; ++q) out[q] = red1p(red1p(u[q] + u[q + 8], p) + u[q + 12], p);
+#pragma unroll
+		for (int q = 4; q < 8; ++q) out[q] = red1p(red1p(u[q] + u[q + 4], p) + red1p(u[q + 8] + u[q + 8], p), p);
+#pragma unroll
+		for (int q = 8; q < E; ++q) out[q] = 0;
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < E; ++k) out[k] = u[k];
+	}
 }
 
-// One squaring (MUL = false, x <- x^2 * 2^dup) or product (MUL = true,
+// One squaring (MUL = false, x <- x^2 * base^dup) or product (MUL = true,
 // x <- x * y) with CRT and the chunked carry, over NP primes (NP = 2 when every
 // coefficient is known to be below p1 p2 / 2, else 3).  x lives in LDS region
 // D (layout 0) before and after; y (MUL only) in registers, layout 0.
-template <int LN, bool MUL, int NP>
-__device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool dup, const divisor &dv,
+template <int LN, bool MUL, int NP, bool MINUS>
+__device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool dup, int base_m1, const divisor &dv,
                                      uint32_t *lds, int64_t *fbuf, const tw_entry *tw, const garner_consts &gc, uint32_t t)
 {
 	constexpr int a = geo<LN>::a;
 	constexpr uint32_t T = geo<LN>::T;
+	constexpr uint32_t TC = MINUS ? T / 2 : T;          // threads that own a chunk of digits
 	uint32_t r0[E], r1[E], r2[E];
-	transform<LN, 0, MUL>(D, yd, r0, lds, tw, t);
-	transform<LN, 1, MUL>(D, yd, r1, lds, tw, t);
-	if (NP == 3) transform<LN, 2, MUL>(D, yd, r2, lds, tw, t);
+	transform<LN, 0, MUL, MINUS>(D, yd, r0, lds, tw, t);
+	transform<LN, 1, MUL, MINUS>(D, yd, r1, lds, tw, t);
+	if (NP == 3) transform<LN, 2, MUL, MINUS>(D, yd, r2, lds, tw, t);
 	xchg<LN, 0, a>(r0, lds, t);
 	xchg<LN, 0, a>(r1, lds, t);
 	if (NP == 3) xchg<LN, 0, a>(r2, lds, t);
 
-	// normalize1: CRT, optional doubling, signed digits of 16 consecutive coefficients
+	// normalize1: CRT, optional multiplication by the base, signed digits of 16
+	// consecutive coefficients
 	int32_t dig[E];
-	if (NP == 3)
+	if (t < TC)
 	{
-		i96 l[E];
+		if (NP == 3)
+		{
+			i96 l[E];
 #pragma unroll
-		for (int q = 0; q < E; ++q) l[q] = garner(r0[q], r1[q], r2[q], gc);
-		fbuf[t] = carry_chunk(l, dig, dup, dv);
-	}
-	else
-	{
-		int64_t l[E];
+			for (int q = 0; q < E; ++q) l[q] = garner(r0[q], r1[q], r2[q], gc);
+			fbuf[t] = carry_chunk(l, dig, dup, base_m1, dv);
+		}
+		else
+		{
+			int64_t l[E];
 #pragma unroll
-		for (int q = 0; q < E; ++q) l[q] = garner2(r0[q], r1[q], gc);
-		fbuf[t] = carry_chunk64(l, dig, dup, dv);
+			for (int q = 0; q < E; ++q) l[q] = garner2(r0[q], r1[q], gc);
+			fbuf[t] = carry_chunk64(l, dig, dup, base_m1, dv);
+		}
 	}
 	__syncthreads();
-	// normalize2: carry in from the previous chunk; X^n = X^(n/2) - 1 wraps the
-	// top carry to chunk 0 (negated) and to chunk T/2 (added)
-	int64_t c = fbuf[(t + T - 1) % T];
-	if (t == 0) c = -c;
-	else if (t == T / 2) c += fbuf[T - 1];
-	carry_in(dig, c, dv);
-	// the carried chunk goes straight back to D (every reader of D in this
-	// step has passed the barriers above)
-	int32_t *dw = D + lbase<LN, a>(t);
+	// normalize2: carry in from the previous chunk; X^n = X^(n/2) - 1 (N+) wraps
+	// the top carry to chunk 0 negated and adds it to chunk TC/2; for N-,
+	// X^n = X^(n/2) + 1 adds it to both
+	if (t < TC)
+	{
+		int64_t c = fbuf[(t + TC - 1) % TC];
+		if (t == 0) c = MINUS ? c : -c;
+		else if (t == TC / 2) c += fbuf[TC - 1];
+		carry_in(dig, c, dv);
+		// the carried chunk goes straight back to D (every reader of D in this
+		// step has passed the barriers above)
+		int32_t *dw = D + lbase<LN, a>(t);
 #pragma unroll
-	for (int q = 0; q < E; ++q) dw[loff<LN, a>(q)] = dig[q];
+		for (int q = 0; q < E; ++q) dw[loff<LN, a>(q)] = dig[q];
+	}
 	__syncthreads();
 }
 
 // ---- kernels -----------------------------------------------------------------
 
-template <int LN, int NP>
+template <int LN, int NP, bool MINUS>
 __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
 {
-	constexpr uint32_t n = geo<LN>::n, T = geo<LN>::T;
+	constexpr uint32_t T = geo<LN>::T, n = MINUS ? geo<LN>::n / 2 : geo<LN>::n;    // digits per candidate
+	constexpr int QD = MINUS ? E / 2 : E;                                           // layout-0 slots holding digits
 	__shared__ uint32_t lds[geo<LN>::lds_words];
 	__shared__ int32_t D[geo<LN>::lds_words];
 	__shared__ int64_t fbuf[T];
@@ -123,7 +156,7 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
 	{
 		int32_t *dw = D + lbase<LN, 0>(t);
 #pragma unroll
-		for (int q = 0; q < E; ++q) dw[loff<LN, 0>(q)] = x[t + q * T];
+		for (int q = 0; q < E; ++q) dw[loff<LN, 0>(q)] = (q < QD) ? x[t + q * T] : 0;
 	}
 	__syncthreads();
 	const divisor dv = A.dv[c];
@@ -136,17 +169,18 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
 	{
 		const int i = i_top - it;
 		const bool dup = bits ? ((bits[i >> 5] >> (i & 31)) & 1u) : mdup;
-		step<LN, false, NP>(D, none, dup, dv, lds, fbuf, A.tw, A.gc, t);
+		step<LN, false, NP, MINUS>(D, none, dup, A.base_m1, dv, lds, fbuf, A.tw, A.gc, t);
 	}
 	const int32_t *dr = D + lbase<LN, 0>(t);
 #pragma unroll
-	for (int q = 0; q < E; ++q) x[t + q * T] = dr[loff<LN, 0>(q)];
+	for (int q = 0; q < QD; ++q) x[t + q * T] = dr[loff<LN, 0>(q)];
 }
 
-template <int LN, int NP>
+template <int LN, int NP, bool MINUS>
 __global__ __launch_bounds__(geo<LN>::T) void k_res_mul(mul_args A)
 {
-	constexpr uint32_t n = geo<LN>::n, T = geo<LN>::T;
+	constexpr uint32_t T = geo<LN>::T, n = MINUS ? geo<LN>::n / 2 : geo<LN>::n;
+	constexpr int QD = MINUS ? E / 2 : E;
 	__shared__ uint32_t lds[geo<LN>::lds_words];
 	__shared__ int32_t D[geo<LN>::lds_words];
 	__shared__ int64_t fbuf[T];
@@ -156,65 +190,82 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_mul(mul_args A)
 	{
 		int32_t *dw = D + lbase<LN, 0>(t);
 #pragma unroll
-		for (int q = 0; q < E; ++q) { dw[loff<LN, 0>(q)] = xa[t + q * T]; yd[q] = xb[t + q * T]; }
+		for (int q = 0; q < E; ++q)
+		{
+			dw[loff<LN, 0>(q)] = (q < QD) ? xa[t + q * T] : 0;
+			yd[q] = (q < QD) ? xb[t + q * T] : 0;
+		}
 	}
 	__syncthreads();
 	const divisor dv = A.dv[c];
-	step<LN, true, NP>(D, yd, false, dv, lds, fbuf, A.tw, A.gc, t);
+	step<LN, true, NP, MINUS>(D, yd, false, 1, dv, lds, fbuf, A.tw, A.gc, t);
 	int32_t *xo = A.out + (size_t)c * n;
 	const int32_t *dr = D + lbase<LN, 0>(t);
 #pragma unroll
-	for (int q = 0; q < E; ++q) xo[t + q * T] = dr[loff<LN, 0>(q)];
+	for (int q = 0; q < QD; ++q) xo[t + q * T] = dr[loff<LN, 0>(q)];
 }
 
 template <int LN>
-static hipError_t launch_steps_ln(int np, const res_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t launch_steps_ln(int np, bool minus, const res_args &A, uint32_t lanes, hipStream_t st)
 {
-	if (np == 2) hipLaunchKernelGGL((k_res_steps<LN, 2>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
-	else hipLaunchKernelGGL((k_res_steps<LN, 3>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
-	return hipGetLastError();
-}
-
-template <int LN>
-static hipError_t launch_mul_ln(int np, const mul_args &A, uint32_t lanes, hipStream_t st)
-{
-	if (np == 2) hipLaunchKernelGGL((k_res_mul<LN, 2>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
-	else hipLaunchKernelGGL((k_res_mul<LN, 3>), dim3(lanes), dim3(geo<LN>::T), 0, st, A);
-	return hipGetLastError();
-}
-
-hipError_t resident_steps(int ln, int np, const res_args &A, uint32_t lanes, hipStream_t st)
-{
-	switch (ln)
+	const dim3 g(lanes), b(geo<LN>::T);
+	if (minus)
 	{
-	case 5: return launch_steps_ln<5>(np, A, lanes, st);
-	case 6: return launch_steps_ln<6>(np, A, lanes, st);
-	case 7: return launch_steps_ln<7>(np, A, lanes, st);
-	case 8: return launch_steps_ln<8>(np, A, lanes, st);
-	case 9: return launch_steps_ln<9>(np, A, lanes, st);
-	case 10: return launch_steps_ln<10>(np, A, lanes, st);
-	case 11: return launch_steps_ln<11>(np, A, lanes, st);
-	case 12: return launch_steps_ln<12>(np, A, lanes, st);
-	case 13: return launch_steps_ln<13>(np, A, lanes, st);
-	case 14: return launch_steps_ln<14>(np, A, lanes, st);
+		if (np == 2) hipLaunchKernelGGL((k_res_steps<LN, 2, true>), g, b, 0, st, A);
+		else hipLaunchKernelGGL((k_res_steps<LN, 3, true>), g, b, 0, st, A);
+	}
+	else if (np == 2) hipLaunchKernelGGL((k_res_steps<LN, 2, false>), g, b, 0, st, A);
+	else hipLaunchKernelGGL((k_res_steps<LN, 3, false>), g, b, 0, st, A);
+	return hipGetLastError();
+}
+
+template <int LN>
+static hipError_t launch_mul_ln(int np, bool minus, const mul_args &A, uint32_t lanes, hipStream_t st)
+{
+	const dim3 g(lanes), b(geo<LN>::T);
+	if (minus)
+	{
+		if (np == 2) hipLaunchKernelGGL((k_res_mul<LN, 2, true>), g, b, 0, st, A);
+		else hipLaunchKernelGGL((k_res_mul<LN, 3, true>), g, b, 0, st, A);
+	}
+	else if (np == 2) hipLaunchKernelGGL((k_res_mul<LN, 2, false>), g, b, 0, st, A);
+	else hipLaunchKernelGGL((k_res_mul<LN, 3, false>), g, b, 0, st, A);
+	return hipGetLastError();
+}
+
+// tln: log2 of the transform length (n digits for N+, 2n for N-)
+hipError_t resident_steps(int tln, int np, bool minus, const res_args &A, uint32_t lanes, hipStream_t st)
+{
+	switch (tln)
+	{
+	case 5: return launch_steps_ln<5>(np, minus, A, lanes, st);
+	case 6: return launch_steps_ln<6>(np, minus, A, lanes, st);
+	case 7: return launch_steps_ln<7>(np, minus, A, lanes, st);
+	case 8: return launch_steps_ln<8>(np, minus, A, lanes, st);
+	case 9: return launch_steps_ln<9>(np, minus, A, lanes, st);
+	case 10: return launch_steps_ln<10>(np, minus, A, lanes, st);
+	case 11: return launch_steps_ln<11>(np, minus, A, lanes, st);
+	case 12: return launch_steps_ln<12>(np, minus, A, lanes, st);
+	case 13: return launch_steps_ln<13>(np, minus, A, lanes, st);
+	case 14: return launch_steps_ln<14>(np, minus, A, lanes, st);
 	default: return hipErrorInvalidValue;
 	}
 }
 
-hipError_t resident_mul(int ln, int np, const mul_args &A, uint32_t lanes, hipStream_t st)
+hipError_t resident_mul(int tln, int np, bool minus, const mul_args &A, uint32_t lanes, hipStream_t st)
 {
-	switch (ln)
+	switch (tln)
 	{
-	case 5: return launch_mul_ln<5>(np, A, lanes, st);
-	case 6: return launch_mul_ln<6>(np, A, lanes, st);
-	case 7: return launch_mul_ln<7>(np, A, lanes, st);
-	case 8: return launch_mul_ln<8>(np, A, lanes, st);
-	case 9: return launch_mul_ln<9>(np, A, lanes, st);
-	case 10: return launch_mul_ln<10>(np, A, lanes, st);
-	case 11: return launch_mul_ln<11>(np, A, lanes, st);
-	case 12: return launch_mul_ln<12>(np, A, lanes, st);
-	case 13: return launch_mul_ln<13>(np, A, lanes, st);
-	case 14: return launch_mul_ln<14>(np, A, lanes, st);
+	case 5: return launch_mul_ln<5>(np, minus, A, lanes, st);
+	case 6: return launch_mul_ln<6>(np, minus, A, lanes, st);
+	case 7: return launch_mul_ln<7>(np, minus, A, lanes, st);
+	case 8: return launch_mul_ln<8>(np, minus, A, lanes, st);
+	case 9: return launch_mul_ln<9>(np, minus, A, lanes, st);
+	case 10: return launch_mul_ln<10>(np, minus, A, lanes, st);
+	case 11: return launch_mul_ln<11>(np, minus, A, lanes, st);
+	case 12: return launch_mul_ln<12>(np, minus, A, lanes, st);
+	case 13: return launch_mul_ln<13>(np, minus, A, lanes, st);
+	case 14: return launch_mul_ln<14>(np, minus, A, lanes, st);
 	default: return hipErrorInvalidValue;
 	}
 }

@@ -290,7 +290,7 @@ int ct_server_step(ct_server *s)
 			if (!(ls.busy && ls.remaining == 0)) continue;
 			if (hipMemcpy(raw.data(), e->reg(0) + (size_t)l * n, n * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
 				return fail(CT_ERR_HIP, "read-out");
-			if (canonical(raw.data(), n, ls.req.b, canon.data())) return fail(CT_ERR_GETINT, "getInt failed");
+			if (canonical(raw.data(), n, ls.req.b, 1, canon.data())) return fail(CT_ERR_GETINT, "getInt failed");
 			uint64_t r64 = canon[0], bi = ls.req.b, h = 1469598103934665603ull;
 			for (uint32_t j = 1; j < n; ++j) { r64 += (uint64_t)canon[j] * bi; bi *= ls.req.b; }
 			for (uint32_t j = 0; j < n; ++j) { h ^= canon[j]; h *= 1099511628211ull; }

@@ -52,6 +52,15 @@ typedef struct ct_engine ct_engine;
  * lanes, kind) for the process.  Allocates 3 digit registers per lane. */
 int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out);
 
+/* ct_engine_create for either twin member: sign = +1 gives the engine above
+ * (N+ = b^n - b^(n/2) + 1), sign = -1 an engine modulo N- = b^n - b^(n/2) - 1
+ * (no PRP engine in the reference; its sieve defines N-, ctsieve.cpp:728-787).
+ * The N- engine computes the exact product with the same transform at length
+ * 2n (digits zero-padded) and folds it by X^n = X^(n/2) + 1; log2_n <= 21, resident
+ * kernels for n <= 2^13.  Every call below works on both kinds of engine; the
+ * context file and the batch driver are N+ only. */
+int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, int sign, ct_engine **out);
+
 /* Multipass engine: columns per workgroup of the column passes (16 or 32 for
  * n >= 2^13; 64 below).  Results do not depend on it.  No counterpart in the
  * reference (its work-group geometry is fixed per kernel, kernel.cl). */
@@ -72,7 +81,13 @@ int ct_engine_set(ct_engine *e, uint32_t a);
 /* engine::copy (engine.h:240-247): register dst := register src (0..2). */
 int ct_engine_copy(ct_engine *e, uint32_t dst, uint32_t src);
 
-/* engine::squareDup (engine.h:387-419): register 0 := x^2 * 2^bit(lane),
+/* PRP base (2, the default and the reference's, or 3): a set bit of the
+ * squaring steps below multiplies by it, so ct_prp_check computes
+ * base^(N-1) mod N.  No counterpart in the reference (base 2 is fixed in its
+ * normalize1, kernel.cl:1044). */
+int ct_engine_set_base(ct_engine *e, uint32_t base);
+
+/* engine::squareDup (engine.h:387-419): register 0 := x^2 * base^bit(lane),
  * bit of lane l = (dup[l / 64] >> (l % 64)) & 1; dup may be NULL (no doubling). */
 int ct_engine_square_dup(ct_engine *e, const uint64_t *dup);
 
@@ -105,13 +120,14 @@ int ct_engine_write_state(ct_engine *e, uint32_t reg, const int32_t *digits);
 int ct_engine_set_exponents(ct_engine *e, const uint32_t *bits, uint32_t words);
 
 /* The PRP loop of cyclo.h:277-308 on the device: for i = i_hi down to i_lo,
- * register 0 := x^2 * 2^(bit i); when gl_block > 0, after the step for bit i
+ * register 0 := x^2 * base^(bit i); when gl_block > 0, after the step for bit i
  * with i % gl_block == 0 and i / gl_block != 0, register 1 := register 0 *
  * register 1 (Gerbicz-Li d(t)). */
 int ct_engine_run(ct_engine *e, int i_hi, int i_lo, int gl_block);
 
-/* cyclo::check for one batch (cyclo.h:252-411): base-2 Fermat PRP of
- * b^n - b^(n/2) + 1 for every lane with Gerbicz-Li verification.  Outputs
+/* cyclo::check for one batch (cyclo.h:252-411): Fermat PRP (the engine's base,
+ * 2 by default) of b^n - b^(n/2) + 1 (or - 1 for an N- engine) for every lane with
+ * Gerbicz-Li verification.  Outputs
  * per lane: prm (probable prime), res64 and gl_ok; the canonical residues
  * (optional, may be NULL) as residue[lane * n + j].  Returns CT_ERR_GERBICZ if
  * any lane fails verification, CT_ERR_COMPUTE if any residue is 0. */

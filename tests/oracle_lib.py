@@ -58,6 +58,9 @@ def lib():
         L.or_sieve_window.restype = c.c_int
         L.or_sieve_factors.argtypes = [c.c_int, c.c_uint64, c.c_uint64, c.c_int, u64p, c.c_uint64, c.POINTER(c.c_uint64)]
         L.or_sieve_factors.restype = c.c_int
+        L.or_powm.argtypes = [c.c_char_p, c.c_size_t, c.c_char_p, c.c_size_t, c.c_char_p, c.c_size_t,
+                              c.c_char_p, c.c_size_t]
+        L.or_powm.restype = c.c_int
         _lib = L
     return _lib
 
@@ -184,11 +187,38 @@ def sieve_factors(m, p_min, p_max, mode, cap=1 << 20):
 
 # ---- pure-Python helpers (independent of the oracle) ----
 
+def powm(a, e, m):
+    """a^e mod m by GMP's mpz_powm (oracle/gmp_powm.c)."""
+    nb = max(1, (a.bit_length() + 7) // 8)
+    ne = max(1, (e.bit_length() + 7) // 8)
+    nm = max(1, (m.bit_length() + 7) // 8)
+    out = ctypes.create_string_buffer(nm)
+    rc = lib().or_powm(a.to_bytes(nb, "little"), nb, e.to_bytes(ne, "little"), ne, m.to_bytes(nm, "little"), nm,
+                       out, nm)
+    assert rc == 0
+    return int.from_bytes(out.raw, "little")
+
+
 def digits_value(digits, b):
-    v = 0
-    for d in reversed([int(x) for x in digits]):
-        v = v * b + d
-    return v
+    """sum d_j b^j (divide and conquer: quasi-linear with Python's big integers)."""
+    ds = [int(x) for x in digits]
+    powers = {}
+
+    def bpow(k):
+        if k not in powers:
+            powers[k] = pow(b, k)
+        return powers[k]
+
+    def rec(lo, hi):
+        if hi - lo <= 64:
+            v = 0
+            for d in reversed(ds[lo:hi]):
+                v = v * b + d
+            return v
+        mid = (lo + hi) // 2
+        return rec(lo, mid) + rec(mid, hi) * bpow(mid - lo)
+
+    return rec(0, len(ds)) if ds else 0
 
 
 def modulus(n, b):

@@ -64,7 +64,7 @@ def _expected_lines(ln, bs):
         if i > 0 and b == bs[i - 1]:
             continue
         N = O.modulus(n, b)
-        r = pow(2, N - 1, N)
+        r = O.powm(2, N - 1, N)
         head = f"{b}^{n} - {b}^{n // 2} + 1 is "
         if r == 1:
             lines.append(head + "a probable prime."); prps.append(b)

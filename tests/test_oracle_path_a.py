@@ -249,3 +249,19 @@ def test_prp_steps_are_powers_of_two():
         d = O.Engine(n, b, 16).prp_steps(t)
         want = pow(2, E >> (top - t + 1), N)
         assert O.digits_value(d, b) % N == want
+
+
+def test_gmp_powm_oracle_matches_python_pow():
+    """The GMP mpz_powm oracle used by the GPU tests for large PRP residues
+    equals Python's pow on random and structured cases."""
+    import random
+    rng = random.Random(5)
+    for _ in range(50):
+        m = rng.randrange(2, 1 << rng.randrange(2, 700)) | 1
+        a, e = rng.randrange(0, 1 << 300), rng.randrange(0, 1 << 600)
+        assert O.powm(a, e, m) == pow(a, e, m)
+    for sign in (1, -1):
+        for b in (1000, 65537, 700001):
+            N = b ** 64 - b ** 32 + sign
+            assert O.powm(3, N - 1, N) == pow(3, N - 1, N)
+    assert O.powm(5, 0, 7) == 1 and O.powm(0, 5, 7) == 0
