@@ -142,7 +142,11 @@ CT_HD i96 i96_negate(i96 x)
 // (N. Moller and T. Granlund, "Improved division by invariant integers",
 // IEEE Trans. Computers 60 (2011), algorithm 4).  Quotient < 2^32.
 
-struct divisor { uint32_t d; uint32_t v; uint32_t sh; uint32_t b; };
+struct divisor
+{
+	uint32_t d; uint32_t v; uint32_t sh; uint32_t b;   // normalised divisor, its 2-by-1 reciprocal, shift, b
+	uint32_t m_lo, m_hi;                               // m = floor((2^64 - 1) / b), the 64-bit reciprocal
+};
 
 CT_HD divisor make_divisor(uint32_t b)
 {
@@ -150,6 +154,8 @@ CT_HD divisor make_divisor(uint32_t b)
 	uint32_t sh = 0; while (((b << sh) & 0x80000000u) == 0) ++sh;
 	r.sh = sh; r.d = b << sh;
 	r.v = (uint32_t)((~(uint64_t)0) / r.d - ((uint64_t)1 << 32));
+	const uint64_t m = (~(uint64_t)0) / b;
+	r.m_lo = (uint32_t)m; r.m_hi = (uint32_t)(m >> 32);
 	return r;
 }
 
@@ -193,12 +199,28 @@ CT_HD int32_t divrem_i96(i96 &a, const divisor &dv)
 	return s ? -(int32_t)r : (int32_t)r;
 }
 
+// |a| < 2^63 divided by b with the 64-bit reciprocal m = floor((2^64-1)/b):
+// q' = floor(|a| m / 2^64) is q or q - 1 (|a| (2^64 - 1)/b - |a| - 1 < |a| m
+// <= |a| 2^64 / b and |a| < 2^63), so one correction step gives the exact
+// quotient; the remainder < 2b < 2^32 comes from the low words alone.
+CT_HD uint64_t udiv64_recip(uint64_t u, const divisor &dv, uint32_t *rem)
+{
+	const uint32_t u0 = (uint32_t)u, u1 = (uint32_t)(u >> 32);
+	const uint64_t s1 = (uint64_t)u0 * dv.m_hi + mulhi32(u0, dv.m_lo);             // < 2^64
+	const uint64_t s2 = (uint64_t)u1 * dv.m_lo + (uint32_t)s1;                       // < 2^64
+	uint64_t q = (uint64_t)u1 * dv.m_hi + ((s1 >> 32) + (s2 >> 32));                 // floor(u m / 2^64)
+	uint32_t r = u0 - (uint32_t)q * dv.b;
+	if (r >= dv.b) { r -= dv.b; q += 1; }
+	*rem = r;
+	return q;
+}
+
 CT_HD int32_t divrem_i64(int64_t &a, const divisor &dv)
 {
 	const bool s = (a < 0);
 	const uint64_t t = s ? (uint64_t)(-a) : (uint64_t)a;
 	uint32_t r;
-	const uint64_t q = udiv96(0, (uint32_t)(t >> 32), (uint32_t)t, dv, &r);
+	const uint64_t q = udiv64_recip(t, dv, &r);
 	a = s ? -(int64_t)q : (int64_t)q;
 	return s ? -(int32_t)r : (int32_t)r;
 }

@@ -1,0 +1,20 @@
+"""The device arithmetic header compiled for the host (g++) and checked against
+exact 128-bit integer arithmetic: truncating division by b (the carry), lazy
+Shoup products, lazy Montgomery products."""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_device_arithmetic_on_host(tmp_path):
+    exe = str(tmp_path / "arith_check")
+    src = os.path.join(HERE, "host", "arith_check.cpp")
+    r = subprocess.run(["g++", "-O2", "-std=c++17", src, "-o", exe], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.fail(r.stderr)
+    out = subprocess.run([exe, "1000000"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout
+    assert out.stdout.strip().startswith("0 mismatches")
