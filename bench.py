@@ -63,7 +63,7 @@ def run_plain(ct, ln, b, steps, warmup, graphs=True):
     from ctwin_amd.exponent import window_value
     n = 1 << ln
     Ew, i0 = window_value(b, n, 2 * steps + warmup + 8)
-    t = ct.Transform(ln, 1, ct.KIND_PLAIN)
+    t = ct.Transform(ln, 1, ct.KIND_PLAIN, csize=0)     # multipass, tile and pass split from the autotuner
     t.set_graphs(graphs)
     t.set_b([b]); t.set_exponents([Ew]); t.set(1)
     t.run(i0, i0 - warmup + 1)
@@ -248,8 +248,9 @@ def main():
                                "base-2 PRP loop, one persistent resident launch",
                    "log2_n": 14, "candidates": 256, "seed": SEED,
                    "engine": "resident" if kind == ct.KIND_RESIDENT else "multipass",
-                   "autotuner_ms_per_step": {"resident": tuning[1][0], "multipass_w16": tuning[1][1],
-                                             "multipass_w32": tuning[1][2]}},
+                   "autotuner_ms_per_step": {"resident": tuning[2][0], "multipass_w16": tuning[2][1],
+                                             "multipass_w32": tuning[2][2], "multipass_w16_split": tuning[2][3],
+                                             "multipass_w32_split": tuning[2][4]}},
         "candidates_per_hour": 3600.0 * sq_s / per_cand,
         "plain": {"squarings_per_s": plain_sq_s, "candidates_per_hour": 3600.0 * plain_sq_s / per_cand,
                   "engine": "plain multipass, one candidate per launch sequence"},

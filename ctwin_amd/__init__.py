@@ -33,7 +33,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
            "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
-           "ct_engine_create_ex", "ct_engine_set_base"]
+           "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split"]
 
 INTERRUPTED = 1
 
@@ -107,7 +107,8 @@ def lib():
             "ct_context_save": [vp, c.c_char_p, i32, i32, c.c_double],
             "ct_context_load": [vp, c.c_char_p, c.POINTER(i32), c.POINTER(i32), c.POINTER(c.c_double)],
             "ct_cyclo_check_file": [i32, u32, c.c_char_p, c.c_char_p, u32, c.c_double, u64, c.POINTER(CycloStats)],
-            "ct_engine_tuning": [vp, c.POINTER(u32), np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
+            "ct_engine_tuning": [vp, c.POINTER(u32), c.POINTER(i32), np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
+            "ct_engine_set_column_split": [vp, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -224,11 +225,16 @@ class Transform:
         _check(lib().ct_engine_set_column_tile(self.h, int(w)))
 
     def tuning(self):
-        """(column tile, {resident, multipass W=16, W=32} ms per squaring; -1 = not timed)."""
-        w = ctypes.c_uint32()
-        ms = np.zeros(3, np.float32)
-        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ms))
-        return int(w.value), [float(x) for x in ms]
+        """(column tile, split of column pass A, {resident, multipass W=16, W=32,
+        W=16 split, W=32 split} ms per squaring; -1 = not timed)."""
+        w, sp = ctypes.c_uint32(), ctypes.c_int()
+        ms = np.zeros(5, np.float32)
+        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ms))
+        return int(w.value), bool(sp.value), [float(x) for x in ms]
+
+    def set_column_split(self, on):
+        """Multipass engine: column pass A split by prime (1) or combined (0)."""
+        _check(lib().ct_engine_set_column_split(self.h, int(bool(on))))
 
     def context_save(self, path, k, i, elapsed=0.0):
         """Checkpoint registers 0 and 1 in the reference's cyclo.ctx format."""

@@ -44,7 +44,8 @@ int ct::hip_fail(hipError_t e, const char *where)
 // ---------------------------------------------------------------------------
 // engine lifetime
 
-static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, uint32_t col_w, int sign, ct_engine **out)
+static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, uint32_t col_w, bool split, int sign,
+                       ct_engine **out)
 {
 	const uint32_t tln = log2_n + (sign < 0 ? 1 : 0);
 	if (kind == CT_KIND_AUTO) kind = (tln <= 14) ? CT_KIND_RESIDENT : CT_KIND_PLAIN;
@@ -60,7 +61,7 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 	ct_engine *e = new ct_engine();
 	e->device = device; e->ln = log2_n; e->n = 1u << log2_n; e->lanes = lanes; e->csize = 16; e->kind = kind;
 	e->sign = (sign < 0) ? -1 : 1; e->tln = tln; e->tn = 1u << tln;
-	e->col_w = col_w;
+	e->col_w = col_w; e->split_a = split;
 	e->gc = make_garner();
 	e->b.assign(lanes, 0);
 	const size_t reg_words = (size_t)lanes * e->n;
@@ -93,8 +94,9 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 // transform length and lane count, times squarings of a synthetic state under
 // every compiled configuration -- resident kernels vs the multipass engine
 // (n <= 2^14; one resident workgroup per candidate loses to the whole-GPU
-// multipass launch when there are few candidates) and the multipass column
-// tile (16 or 32 columns per workgroup) -- and keeps the fastest.  Results are
+// multipass launch when there are few candidates), the multipass column tile
+// (16 or 32 columns per workgroup) and column pass A combined or split by
+// prime -- and keeps the fastest.  Results are
 // cached per (device, log2 n, lanes) for the life of the process.
 
 namespace {
@@ -102,7 +104,7 @@ namespace {
 struct tune_key { int device; uint32_t ln, lanes; int kind, sign;
 	bool operator<(const tune_key &o) const
 	{ return std::tie(device, ln, lanes, kind, sign) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign); } };
-struct tune_val { int kind; uint32_t col_w; float ms[3]; };
+struct tune_val { int kind; uint32_t col_w; bool split; float ms[5]; };
 std::mutex g_tune_mu;
 std::map<tune_key, tune_val> g_tune;
 
@@ -132,14 +134,14 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 		auto it = g_tune.find(key);
 		if (it != g_tune.end()) { *out = it->second; return CT_OK; }
 	}
-	tune_val v{CT_KIND_AUTO, 0, {-1, -1, -1}};
+	tune_val v{CT_KIND_AUTO, 0, false, {-1, -1, -1, -1, -1}};
 	float best = 1e30f;
 	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && tln <= 14);
 	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && tln >= (sign < 0 ? 11u : 10u));
 	if (try_res)
 	{
 		ct_engine *e = nullptr;
-		int rc = create_impl(device, ln, lanes, CT_KIND_RESIDENT, 0, sign, &e);
+		int rc = create_impl(device, ln, lanes, CT_KIND_RESIDENT, 0, false, sign, &e);
 		if (rc) return rc;
 		v.ms[0] = time_squarings(e);
 		ct_engine_destroy(e);
@@ -149,18 +151,19 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 	if (try_plain)
 	{
 		ct_engine *e = nullptr;
-		int rc = create_impl(device, ln, lanes, CT_KIND_PLAIN, 0, sign, &e);
+		int rc = create_impl(device, ln, lanes, CT_KIND_PLAIN, 0, false, sign, &e);
 		if (rc) return rc;
 		const uint32_t tiles[2] = {16, 32};
-		for (int t = 0; t < 2; ++t)
-		{
-			if (!plain_tile_ok((int)tln, tiles[t])) continue;
-			e->col_w = tiles[t];
-			const float ms = time_squarings(e);
-			if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
-			v.ms[1 + t] = ms;
-			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; }
-		}
+		for (int sp = 0; sp < 2; ++sp)
+			for (int t = 0; t < 2; ++t)
+			{
+				if (!plain_tile_ok((int)tln, tiles[t])) continue;
+				e->col_w = tiles[t]; e->split_a = (sp != 0);
+				const float ms = time_squarings(e);
+				if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
+				v.ms[1 + t + 2 * sp] = ms;
+				if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = (sp != 0); }
+			}
 		if (!plain_tile_ok((int)tln, 16) && !plain_tile_ok((int)tln, 32))
 		{   // small n: one tile only
 			const float ms = time_squarings(e);
@@ -193,11 +196,11 @@ int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t cs
 	if (kind == CT_KIND_RESIDENT && tln > 14) return fail(CT_ERR_UNSUPPORTED, "resident kernels need a transform length <= 2^14");
 	if (kind == CT_KIND_PLAIN && (tln < 10 || (sign < 0 && tln < 11)))
 		return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
-	if (csize == 16) return create_impl(device, log2_n, lanes, kind, 0, sign, out);
+	if (csize == 16) return create_impl(device, log2_n, lanes, kind, 0, false, sign, out);
 	tune_val v;
 	int rc = tune(device, log2_n, lanes, kind, sign, &v);
 	if (rc) return rc;
-	rc = create_impl(device, log2_n, lanes, v.kind, v.col_w, sign, out);
+	rc = create_impl(device, log2_n, lanes, v.kind, v.col_w, v.split, sign, out);
 	if (rc == CT_OK) std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
 	return rc;
 }
@@ -216,11 +219,20 @@ int ct_engine_set_column_tile(ct_engine *e, uint32_t w)
 	return CT_OK;
 }
 
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, float *ms3)
+int ct_engine_set_column_split(ct_engine *e, int on)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (e->kind != CT_KIND_PLAIN) return fail(CT_ERR_UNSUPPORTED, "column passes belong to the multipass engine");
+	e->split_a = (on != 0);
+	return CT_OK;
+}
+
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, float *ms5)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
 	if (col_w) *col_w = (e->kind == CT_KIND_PLAIN) ? e->col_w : 0;
-	if (ms3) for (int i = 0; i < 3; ++i) ms3[i] = e->tune_ms[i];
+	if (split) *split = (e->kind == CT_KIND_PLAIN && e->split_a) ? 1 : 0;
+	if (ms5) for (int i = 0; i < 5; ++i) ms5[i] = e->tune_ms[i];
 	return CT_OK;
 }
 

@@ -33,7 +33,10 @@ struct ct_engine
 	int64_t *d_carry = nullptr;       // [lanes][n/16] chunk carries
 	uint32_t rows_log2 = 0;           // column length R = 2^rows_log2
 	uint32_t col_w = 0;               // column-pass tile width (0 = default for the length)
-	float tune_ms[3] = {-1, -1, -1};  // autotuner timings: resident, multipass W=16, W=32 (ms per squaring)
+	bool split_a = false;             // column pass A: one workgroup per column group and prime
+	// autotuner timings (ms per squaring): resident, multipass W=16, W=32, and
+	// the same two tiles with column pass A split by prime
+	float tune_ms[5] = {-1, -1, -1, -1, -1};
 	bool pending = false;             // register 0 still owes the chunk carries (plain engine)
 	int np = 3;                       // primes per squaring: 2 when 3 n D^2 < p1 p2 / 2
 	uint64_t b_max = 0;               // largest b over the lanes
@@ -42,7 +45,7 @@ struct ct_engine
 	uint32_t base = 2;                // PRP base: a set exponent bit multiplies by it (2 or 3)
 	// captured squaring sequences of the multipass engine (hipGraph), keyed
 	// by what the captured kernel arguments depend on
-	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
+	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool split, first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
 	std::vector<graph_entry> graphs;
 	bool use_graphs = true;
 	int32_t *d_run = nullptr;         // {top bit index of the run, squarings done by replayed graphs}

@@ -44,6 +44,7 @@ struct plain_args
 	const int32_t *lane_i;      // per-lane top bit index (or null, then i0); negative = idle lane
 	int step;                   // squaring index within this launch sequence: bit = top - (iter + step)
 	int base_m1;                // PRP base - 1 (a set bit multiplies by the base)
+	int split_a;                // pass A: one workgroup per column group and prime
 	uint32_t lanes;
 	garner_consts gc;
 };
@@ -98,9 +99,10 @@ __device__ __forceinline__ void col_inv(uint32_t (&u)[E], uint32_t *X, const tw_
 	}
 }
 
+// only: the prime this workgroup transforms (-1: all of them)
 template <int LN, int RL, int NP, int W>
 __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, uint32_t *res_lane, const tw_entry *tw,
-                                               uint32_t c0, uint32_t cw, uint32_t tt, size_t plane)
+                                               uint32_t c0, uint32_t cw, uint32_t tt, size_t plane, int only)
 {
 	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL;
@@ -108,6 +110,7 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 #pragma unroll
 	for (int P = 0; P < NP; ++P)
 	{
+		if (only >= 0 && P != only) continue;
 		uint32_t u[E];
 		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
 		const uint32_t p = kP[P];
@@ -129,7 +132,9 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 // A: chunk load (+ pending carry), column levels, residue planes.  MINUS: the
 // transform has length 2n (n = nd digits in rows < R/2, zeros below), for the
 // exact product modulo N- = b^nd - b^(nd/2) - 1 (folded in pass C).
-template <int LN, int RL, int NP, int W, bool MINUS>
+// SPLIT: one workgroup per column group and prime (blockIdx.z), twice the
+// workgroups of the combined form for the same work (both load the digits).
+template <int LN, int RL, int NP, int W, bool MINUS, bool SPLIT>
 __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_args A, int operand)
 {
 	using G = colgeo<LN, RL, W>;
@@ -171,7 +176,7 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_arg
 	}
 	__syncthreads();
 	const uint32_t cw = t % G::W, tt = t / G::W;
-	colA_prime_all<LN, RL, NP, W>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n);
+	colA_prime_all<LN, RL, NP, W>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n, SPLIT ? (int)blockIdx.z : -1);
 }
 
 // B: row levels with the pointwise square / product, in place
@@ -361,7 +366,8 @@ static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
 	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
+	if (A.split_a) hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, true>), dim3(C / G::W, lanes, NP), dim3(G::threads), 0, st, A, 0);
+	else hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, false>), gcol, dim3(G::threads), 0, st, A, 0);
 	hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
 	hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
 	return hipGetLastError();
@@ -373,8 +379,8 @@ static hipError_t mul_ln(const plain_args &In, const plain_args &Out, uint32_t l
 	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, In, 0);
-	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, In, 1);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, false>), gcol, dim3(G::threads), 0, st, In, 0);
+	hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, false>), gcol, dim3(G::threads), 0, st, In, 1);
 	hipLaunchKernelGGL((k_rowB<LN, RL, true>), dim3(R, NP, lanes), dim3(C / 16), 0, st, Out);
 	hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, Out, 1);
 	return hipGetLastError();
@@ -488,6 +494,7 @@ static plain_args make_args(ct_engine *e)
 	std::memset(&A, 0, sizeof(A));
 	A.res = e->d_res; A.res2 = e->d_res2; A.carry = e->d_carry;
 	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc; A.run = e->d_run; A.base_m1 = (int)e->base - 1;
+	A.split_a = e->split_a ? 1 : 0;
 	return A;
 }
 
@@ -513,7 +520,8 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 {
 	const bool fc = e->pending;
 	for (const auto &g : e->graphs)
-		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.base == e->base && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
+		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.base == e->base && g.split == e->split_a &&
+		    g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
 		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
 	if (e->graphs.size() >= kGraphCache) drop_graphs(e);
 	hipGraph_t graph = nullptr;
@@ -540,7 +548,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	hipGraphDestroy(graph);
 	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
-	e->graphs.push_back({len, e->np, e->col_w, e->base, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
+	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
 	*out = exec;
 	return CT_OK;
 }

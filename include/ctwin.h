@@ -66,10 +66,16 @@ int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t cs
  * reference (its work-group geometry is fixed per kernel, kernel.cl). */
 int ct_engine_set_column_tile(ct_engine *e, uint32_t w);
 
+/* Multipass engine: column pass A as one workgroup per column group (on = 0)
+ * or one per column group and prime (on = 1: twice the workgroups, both load
+ * the digits).  Results do not depend on it.  No counterpart in the reference. */
+int ct_engine_set_column_split(ct_engine *e, int on);
+
 /* What the autotuner chose and measured: the column tile (0 for resident
- * kernels) and ms per squaring for {resident, multipass W=16, multipass W=32}
- * (-1 where not timed or when the engine was not tuned). */
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, float *ms3);
+ * kernels), the split of column pass A, and ms per squaring for {resident,
+ * multipass W=16, W=32, W=16 split, W=32 split} (-1 where not timed or when
+ * the engine was not tuned). */
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, float *ms5);
 int ct_engine_destroy(ct_engine *e);
 
 /* transform::set_b (transform.h:472-486): b[lanes], 32 < b < 2^31. */

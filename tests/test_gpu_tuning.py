@@ -28,15 +28,17 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     bs = [rng.randrange(1 << 17, 1 << 20) for _ in range(lanes)]
     print(f"\n[case] tuned ln={ln} lanes={lanes}", flush=True)
     tuned = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
-    w, ms = tuned.tuning()
+    w, split, ms = tuned.tuning()
     timed = [m for m in ms if m > 0]
     assert timed, ms
     # the chosen configuration is the fastest one timed
     best = min(timed)
     if tuned.kind == ct.KIND_RESIDENT:
-        assert ms[0] == best and w == 0
+        assert ms[0] == best and w == 0 and not split
     else:
-        assert w in (16, 32, 64) and (ms[1] == best or ms[2] == best)
+        assert w in (16, 32, 64)
+        idx = (1 if w in (16, 64) else 2) + (2 if split else 0)
+        assert ms[idx] == best
     ref = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=16)
     a = _run(tuned, bs, n, 40, 1)
     b = _run(ref, bs, n, 40, 1)
@@ -54,8 +56,11 @@ def test_column_tiles_give_identical_digits(ln):
     print(f"\n[case] tiles ln={ln}", flush=True)
     out = []
     for w in (16, 32):
-        t = ct.Transform(ln, 1, ct.KIND_PLAIN)
-        t.set_column_tile(w)
-        assert t.tuning()[0] == w
-        out.append(_run(t, bs, n, 30, 2))
-    assert np.array_equal(out[0], out[1])
+        for split in (False, True):
+            t = ct.Transform(ln, 1, ct.KIND_PLAIN)
+            t.set_column_tile(w)
+            t.set_column_split(split)
+            assert t.tuning()[:2] == (w, split)
+            out.append(_run(t, bs, n, 30, 2))
+    for o in out[1:]:
+        assert np.array_equal(out[0], o)
