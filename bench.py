@@ -23,6 +23,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_T = 39.3          # 256 CU x 64 lanes x 2.4 GHz, 32-bit ops (mul_hi/mul_lo full rate on gfx950)
 SEED = 20261203
+TUNE_NAMES = ("resident", "multipass_w16", "multipass_w32", "multipass_w16_split", "multipass_w32_split",
+              "multipass_w16_fused", "multipass_w32_fused")
 
 
 def mul32_alg(n):
@@ -72,8 +74,10 @@ def run_plain(ct, ln, b, steps, warmup, graphs=True):
     t.timer_start()
     t.run(i, i - steps + 1)
     ms = t.timer_stop()
+    w, split, fused, _ = t.tuning()
+    t.read_state(0)                 # the read checks the fused pass's carry exchanges
     t.close()
-    return steps / (ms / 1e3), ms
+    return steps / (ms / 1e3), ms, {"column_tile": w, "split_pass_a": split, "fused_pass_ca": fused}
 
 
 def run_sieve_config2(ct):
@@ -214,10 +218,10 @@ def main():
 
     sq_s, ms, i0, bs, kind, tuning = run_batched(ct, args.steps, args.warmup)
     print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps)", flush=True)
-    plain_sq_s, plain_ms = run_plain(ct, 14, bs[0], args.plain_steps, 5, graphs=not args.no_graphs)
+    plain_sq_s, plain_ms, plain_cfg = run_plain(ct, 14, bs[0], args.plain_steps, 5, graphs=not args.no_graphs)
     print(f"[bench] plain n=2^14 x1: {plain_sq_s:.4g} squarings/s", flush=True)
     big_b = 65536 + 2 * random.Random(SEED).randrange(1 << 14)
-    big_sq_s, big_ms = run_plain(ct, 21, big_b, args.big_steps, 5, graphs=not args.no_graphs)
+    big_sq_s, big_ms, big_cfg = run_plain(ct, 21, big_b, args.big_steps, 5, graphs=not args.no_graphs)
     print(f"[bench] plain n=2^21 x1: {big_sq_s:.4g} squarings/s", flush=True)
 
     sieve = run_sieve_config2(ct)
@@ -248,14 +252,13 @@ def main():
                                "base-2 PRP loop, one persistent resident launch",
                    "log2_n": 14, "candidates": 256, "seed": SEED,
                    "engine": "resident" if kind == ct.KIND_RESIDENT else "multipass",
-                   "autotuner_ms_per_step": {"resident": tuning[2][0], "multipass_w16": tuning[2][1],
-                                             "multipass_w32": tuning[2][2], "multipass_w16_split": tuning[2][3],
-                                             "multipass_w32_split": tuning[2][4]}},
+                   "autotuner_ms_per_step": dict(zip(TUNE_NAMES, tuning[3]))},
         "candidates_per_hour": 3600.0 * sq_s / per_cand,
         "plain": {"squarings_per_s": plain_sq_s, "candidates_per_hour": 3600.0 * plain_sq_s / per_cand,
-                  "engine": "plain multipass, one candidate per launch sequence"},
+                  "engine": "plain multipass, one candidate per launch sequence", "tuned": plain_cfg},
         "batched_vs_plain": sq_s / plain_sq_s,
         "single_large": {"log2_n": 21, "b": big_b, "squarings_per_s": big_sq_s, "ms_per_squaring": big_ms / args.big_steps,
+                         "tuned": big_cfg,
                          "roofline": {"bound": "hbm", "achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": hbm_achieved / HBM_PEAK_GBS, "traffic": None,
                                       "bytes_alg_per_squaring": bytes_alg(big_n)}},

@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libctwin_amd.so")
 
 KIND_AUTO, KIND_RESIDENT, KIND_PLAIN = 0, 1, 2
+TUNE_TIMINGS = 7                     # CT_TUNE_TIMINGS
 
 STATUS = {0: "CT_OK", -1: "CT_ERR_ARG", -2: "CT_ERR_HIP", -3: "CT_ERR_NOMEM", -4: "CT_ERR_UNSUPPORTED",
           -5: "CT_ERR_GETINT", -6: "CT_ERR_COMPUTE", -7: "CT_ERR_GERBICZ"}
@@ -33,7 +34,8 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
            "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
-           "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split"]
+           "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split",
+           "ct_engine_set_fused"]
 
 INTERRUPTED = 1
 
@@ -107,8 +109,10 @@ def lib():
             "ct_context_save": [vp, c.c_char_p, i32, i32, c.c_double],
             "ct_context_load": [vp, c.c_char_p, c.POINTER(i32), c.POINTER(i32), c.POINTER(c.c_double)],
             "ct_cyclo_check_file": [i32, u32, c.c_char_p, c.c_char_p, u32, c.c_double, u64, c.POINTER(CycloStats)],
-            "ct_engine_tuning": [vp, c.POINTER(u32), c.POINTER(i32), np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
+            "ct_engine_tuning": [vp, c.POINTER(u32), c.POINTER(i32), c.POINTER(i32),
+                                 np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
             "ct_engine_set_column_split": [vp, i32],
+            "ct_engine_set_fused": [vp, i32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -225,12 +229,19 @@ class Transform:
         _check(lib().ct_engine_set_column_tile(self.h, int(w)))
 
     def tuning(self):
-        """(column tile, split of column pass A, {resident, multipass W=16, W=32,
-        W=16 split, W=32 split} ms per squaring; -1 = not timed)."""
-        w, sp = ctypes.c_uint32(), ctypes.c_int()
-        ms = np.zeros(5, np.float32)
-        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ms))
-        return int(w.value), bool(sp.value), [float(x) for x in ms]
+        """(column tile, split of column pass A, fused column pass, ms per
+        squaring for {resident, multipass W=16, W=32, W=16 split, W=32 split,
+        W=16 fused, W=32 fused}; -1 = not timed)."""
+        w, sp, fu = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(TUNE_TIMINGS, np.float32)
+        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ctypes.byref(fu), ms))
+        return int(w.value), bool(sp.value), bool(fu.value), [float(x) for x in ms]
+
+    def set_fused(self, on):
+        """Multipass engine (N+, n >= 2^13): column pass C fused with the next
+        pass A (1, or 2: workgroups always ordered by ticket) or three
+        launches per squaring (0)."""
+        _check(lib().ct_engine_set_fused(self.h, int(on)))
 
     def set_column_split(self, on):
         """Multipass engine: column pass A split by prime (1) or combined (0)."""

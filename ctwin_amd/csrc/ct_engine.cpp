@@ -95,8 +95,9 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 // every compiled configuration -- resident kernels vs the multipass engine
 // (n <= 2^14; one resident workgroup per candidate loses to the whole-GPU
 // multipass launch when there are few candidates), the multipass column tile
-// (16 or 32 columns per workgroup) and column pass A combined or split by
-// prime -- and keeps the fastest.  Results are
+// (16 or 32 columns per workgroup), column pass A combined or split by prime
+// and, for N+ with 256-row columns, pass C fused with the next pass A -- and
+// keeps the fastest.  Results are
 // cached per (device, log2 n, lanes) for the life of the process.
 
 namespace {
@@ -104,23 +105,31 @@ namespace {
 struct tune_key { int device; uint32_t ln, lanes; int kind, sign;
 	bool operator<(const tune_key &o) const
 	{ return std::tie(device, ln, lanes, kind, sign) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign); } };
-struct tune_val { int kind; uint32_t col_w; bool split; float ms[5]; };
+struct tune_val { int kind; uint32_t col_w; bool split, fused; float ms[CT_TUNE_TIMINGS]; };
 std::mutex g_tune_mu;
 std::map<tune_key, tune_val> g_tune;
 
-constexpr int kTuneWarm = 4, kTuneSteps = 24;
+constexpr int kTuneSteps = 24, kTuneReps = 2;
 
 // ms per squaring of engine e (3 primes, b = 2^30 + 9699690 i as in
-// transform.h:391 but below 2^31), or a negative value on failure.
+// transform.h:391 but below 2^31), or a negative value on failure: the
+// fastest of kTuneReps timed runs of kTuneSteps squarings after a warm-up run
+// of the same length (which also captures the graph of that length).
 float time_squarings(ct_engine *e)
 {
 	std::vector<uint32_t> b(e->lanes);
 	for (uint32_t i = 0; i < e->lanes; ++i) b[i] = (1u << 30) + 9699690u * (i % 100);
 	if (ct_engine_set_b(e, b.data()) || ct_engine_set(e, 3)) return -1;
-	if (ct::steps(e, nullptr, 0, 0, kTuneWarm, nullptr)) return -1;
-	float ms = 0;
-	if (ct_engine_timer_start(e) || ct::steps(e, nullptr, 0, 0, kTuneSteps, nullptr) || ct_engine_timer_stop(e, &ms)) return -1;
-	return ms / kTuneSteps;
+	if (ct::steps(e, nullptr, 0, 0, kTuneSteps, nullptr)) return -1;
+	float best = 1e30f;
+	for (int r = 0; r < kTuneReps; ++r)
+	{
+		float ms = 0;
+		if (ct_engine_timer_start(e) || ct::steps(e, nullptr, 0, 0, kTuneSteps, nullptr) || ct_engine_timer_stop(e, &ms))
+			return -1;
+		best = std::min(best, ms);
+	}
+	return best / kTuneSteps;
 }
 
 }   // namespace
@@ -134,7 +143,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 		auto it = g_tune.find(key);
 		if (it != g_tune.end()) { *out = it->second; return CT_OK; }
 	}
-	tune_val v{CT_KIND_AUTO, 0, false, {-1, -1, -1, -1, -1}};
+	tune_val v{CT_KIND_AUTO, 0, false, false, {-1, -1, -1, -1, -1, -1, -1}};
 	float best = 1e30f;
 	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && tln <= 14);
 	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && tln >= (sign < 0 ? 11u : 10u));
@@ -162,7 +171,20 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 				const float ms = time_squarings(e);
 				if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
 				v.ms[1 + t + 2 * sp] = ms;
-				if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = (sp != 0); }
+				if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = (sp != 0); v.fused = false; }
+			}
+		if (sign > 0 && tln >= 13)
+			for (int t = 0; t < 2; ++t)
+			{
+				if (!plain_tile_ok((int)tln, tiles[t])) continue;
+				e->col_w = tiles[t]; e->split_a = false; e->use_fused = true;
+				float ms = time_squarings(e);
+				// a failed exchange surfaces at the next read of register 0
+				if (ms >= 0 && plain_materialize(e) != CT_OK) ms = -1;
+				e->use_fused = false;
+				if (ms < 0) continue;              // not a candidate
+				v.ms[5 + t] = ms;
+				if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = false; v.fused = true; }
 			}
 		if (!plain_tile_ok((int)tln, 16) && !plain_tile_ok((int)tln, 32))
 		{   // small n: one tile only
@@ -201,7 +223,11 @@ int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t cs
 	int rc = tune(device, log2_n, lanes, kind, sign, &v);
 	if (rc) return rc;
 	rc = create_impl(device, log2_n, lanes, v.kind, v.col_w, v.split, sign, out);
-	if (rc == CT_OK) std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
+	if (rc == CT_OK)
+	{
+		std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
+		(*out)->use_fused = v.fused;
+	}
 	return rc;
 }
 
@@ -219,6 +245,17 @@ int ct_engine_set_column_tile(ct_engine *e, uint32_t w)
 	return CT_OK;
 }
 
+int ct_engine_set_fused(ct_engine *e, int on)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (e->kind != CT_KIND_PLAIN) return fail(CT_ERR_UNSUPPORTED, "column passes belong to the multipass engine");
+	if (on < 0 || on > 2) return fail(CT_ERR_ARG, "on must be 0, 1 or 2");
+	e->use_fused = (on != 0);
+	e->fused_force_ticket = (on == 2);
+	e->fused_w = 0;                    // start from cleared flags
+	return CT_OK;
+}
+
 int ct_engine_set_column_split(ct_engine *e, int on)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
@@ -227,12 +264,13 @@ int ct_engine_set_column_split(ct_engine *e, int on)
 	return CT_OK;
 }
 
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, float *ms5)
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, float *ms)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
 	if (col_w) *col_w = (e->kind == CT_KIND_PLAIN) ? e->col_w : 0;
 	if (split) *split = (e->kind == CT_KIND_PLAIN && e->split_a) ? 1 : 0;
-	if (ms5) for (int i = 0; i < 5; ++i) ms5[i] = e->tune_ms[i];
+	if (fused) *fused = (e->kind == CT_KIND_PLAIN && e->use_fused) ? 1 : 0;
+	if (ms) for (int i = 0; i < CT_TUNE_TIMINGS; ++i) ms[i] = e->tune_ms[i];
 	return CT_OK;
 }
 

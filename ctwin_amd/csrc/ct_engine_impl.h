@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "../../include/ctwin.h"
 #include "ct_arith.h"
 #include "ct_tables.h"
 
@@ -34,9 +35,18 @@ struct ct_engine
 	uint32_t rows_log2 = 0;           // column length R = 2^rows_log2
 	uint32_t col_w = 0;               // column-pass tile width (0 = default for the length)
 	bool split_a = false;             // column pass A: one workgroup per column group and prime
+	bool use_fused = false;           // fuse column pass C with the next pass A
+	uint32_t fused_w = 0;             // tile of the grid that last used the flags (0: clear them)
+	int fused_np = 0;                 // and its prime count
+	bool fused_ticket = false;        // the grid does not fit the device at once: order workgroups by ticket
+	bool fused_force_ticket = false;  // order by ticket whether or not it fits
+	bool fused_check = false;         // fused runs since the last read: check the error word
+	size_t fused_bytes = 0;           // flags + {error word, ticket counter, epoch base}
+	int32_t *d_flags = nullptr, *d_ctl = nullptr;   // fused pass: flags per (lane, column group); d_ctl = {error, ticket, epoch}
+	int64_t *d_xc = nullptr;          // fused pass: last-chunk carries per (lane, column group, row)
 	// autotuner timings (ms per squaring): resident, multipass W=16, W=32, and
 	// the same two tiles with column pass A split by prime
-	float tune_ms[5] = {-1, -1, -1, -1, -1};
+	float tune_ms[CT_TUNE_TIMINGS] = {-1, -1, -1, -1, -1, -1, -1};
 	bool pending = false;             // register 0 still owes the chunk carries (plain engine)
 	int np = 3;                       // primes per squaring: 2 when 3 n D^2 < p1 p2 / 2
 	uint64_t b_max = 0;               // largest b over the lanes
@@ -45,7 +55,7 @@ struct ct_engine
 	uint32_t base = 2;                // PRP base: a set exponent bit multiplies by it (2 or 3)
 	// captured squaring sequences of the multipass engine (hipGraph), keyed
 	// by what the captured kernel arguments depend on
-	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool split, first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
+	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool split, fused, first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
 	std::vector<graph_entry> graphs;
 	bool use_graphs = true;
 	int32_t *d_run = nullptr;         // {top bit index of the run, squarings done by replayed graphs}

@@ -237,17 +237,19 @@ __device__ __forceinline__ uint32_t fold_minus(const uint32_t *S, uint32_t col_b
 	return red1p(red1p(at(r) + at(r + R / 4), p) + red1p(c2 + c2, p), p);
 }
 
+// The inverse column levels, the CRT and normalize1 of column pass C for the
+// chunk role of thread t (row r, chunk h of the group's W columns): the
+// chunk's digits and outgoing carry.  Returns false for threads whose row
+// holds no digits (MINUS, r >= R/2).  Every thread passes the barriers.
 template <int LN, int RL, int NP, int W, bool MINUS>
-__global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_args A, int no_dup)
+__device__ __forceinline__ bool colC_chunk(const plain_args &A, uint32_t (*S)[colgeo<LN, RL, W>::W * colgeo<LN, RL, W>::RP],
+                                           uint32_t *X, uint32_t lane, uint32_t c0, uint32_t t, int no_dup,
+                                           int32_t (&dig)[E], int64_t &f)
 {
 	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t n = 1u << LN, C = n >> RL;
-	constexpr uint32_t nd = MINUS ? n / 2 : n, nc = nd / 16, RD = MINUS ? G::R / 2 : G::R;
+	constexpr uint32_t RD = MINUS ? G::R / 2 : G::R;
 	constexpr int La = G::passes - 1;
-	__shared__ uint32_t S[3][G::W * G::RP];
-	__shared__ uint32_t X[G::W * G::RP];
-	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
-	if (A.lane_i && A.lane_i[lane] < 0) return;
 	const uint32_t cw = t % G::W, tt = t / G::W;
 	const size_t plane = (size_t)A.lanes * n;
 	const uint32_t *res = A.res + (size_t)lane * n;
@@ -279,8 +281,7 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 	// chunk role (threads whose row holds digits)
 	constexpr uint32_t hpr = G::W / 16;
 	const uint32_t r = t / hpr, h = t % hpr;
-	if (r >= RD) return;
-	const uint32_t k = r * C + c0 + 16 * h;
+	if (r >= RD) return false;
 	const uint32_t rr = r + (r >> 5);
 	bool dup = false;
 	if (!no_dup)
@@ -289,8 +290,6 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 		if (A.bits) dup = (A.bits[(size_t)lane * A.bit_words + (bi >> 5)] >> (bi & 31)) & 1u;
 		else if (A.mask) dup = (A.mask[lane >> 5] >> (lane & 31)) & 1u;
 	}
-	int32_t dig[E];
-	int64_t f;
 	if (NP == 3)
 	{
 		i96 l[E];
@@ -300,8 +299,8 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 			const uint32_t cb = (16 * h + q) * G::RP, a = cb + rr;
 			if constexpr (MINUS)
 				l[q] = garner(fold_minus<0>(S[0], cb, r, G::R), fold_minus<1>(S[1], cb, r, G::R),
-				              fold_minus<2>(S[2], cb, r, G::R), A.gc);
-			else l[q] = garner(S[0][a], S[1][a], S[2][a], A.gc);
+				              fold_minus<2>(S[NP - 1], cb, r, G::R), A.gc);
+			else l[q] = garner(S[0][a], S[1][a], S[NP - 1][a], A.gc);
 		}
 		f = carry_chunk(l, dig, dup, A.base_m1, A.dv[lane]);
 	}
@@ -317,10 +316,133 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 		}
 		f = carry_chunk64(l, dig, dup, A.base_m1, A.dv[lane]);
 	}
+	return true;
+}
+
+template <int LN, int RL, int NP, int W, bool MINUS>
+__global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_args A, int no_dup)
+{
+	using G = colgeo<LN, RL, W>;
+	constexpr uint32_t n = 1u << LN, C = n >> RL;
+	constexpr uint32_t nd = MINUS ? n / 2 : n, nc = nd / 16, RD = MINUS ? G::R / 2 : G::R;
+	__shared__ uint32_t S[NP][G::W * G::RP];
+	__shared__ uint32_t X[G::W * G::RP];
+	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
+	if (A.lane_i && A.lane_i[lane] < 0) return;
+	int32_t dig[E];
+	int64_t f;
+	if (!colC_chunk<LN, RL, NP, W, MINUS>(A, S, X, lane, c0, t, no_dup, dig, f)) return;
+	constexpr uint32_t hpr = G::W / 16;
+	const uint32_t r = t / hpr, h = t % hpr;
+	const uint32_t k = r * C + c0 + 16 * h;
 	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * nd + k);
 #pragma unroll
 	for (int v = 0; v < 4; ++v) dst[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
 	A.carry[(size_t)lane * nc + cslot(k / 16, C / 16, RD)] = f;
+}
+
+// Column pass C of one squaring fused with column pass A of the next (N+):
+// the inverse column levels, CRT and normalize1 as in k_colC, then the chunk
+// carries cross to the next column group (normalize2 needs the carry of the
+// chunk before each chunk, kernel.cl:1053-1081), then the digits go to HBM and,
+// through LDS, straight into the forward column levels of the next squaring.
+//
+// Exchange.  Group j publishes the carries of its last chunk per row in xc,
+// then raises flags[j] to this launch's epoch and waits until flags[j - 1]
+// holds it (group 0 waits for the last group, whose rows feed the next row's
+// first chunk and, with X^n = X^(n/2) - 1, chunks 0 and n/32).  Every group
+// publishes before it waits, so the waits end once every group has started:
+// when the grid fits the device at once its workgroups are named by blockIdx;
+// otherwise (use_ticket) each takes a ticket (ctl[1], one atomic add; the
+// holder of the last ticket resets the counter) and the ticket names its
+// (lane, column group), so a group waits only for groups that started before
+// it, except group 0, which waits for the last ticket of its lane.  The epoch
+// is ctl[2] + step + 1: ctl[2] advances by the length of each launch
+// sequence, so no two launches share an epoch and flags left by an earlier
+// launch never match.  A wait that exceeds its bound sets ctl[0] and the run
+// is reported as failed.
+template <int LN, int RL, int NP, int W>
+__global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_args A, int32_t *flags, int32_t *ctl,
+                                                                       int64_t *xc, int use_ticket)
+{
+	using G = colgeo<LN, RL, W>;
+	constexpr uint32_t n = 1u << LN, C = n >> RL, GW = C / G::W, hpr = G::W / 16, R = G::R;
+	__shared__ uint32_t S[NP][G::W * G::RP];
+	__shared__ uint32_t X[G::W * G::RP];
+	__shared__ int64_t fl[G::threads];
+	__shared__ uint32_t ticket;
+	const uint32_t t = threadIdx.x;
+	uint32_t grp = blockIdx.x;
+	if (use_ticket)
+	{
+		if (t == 0)
+		{
+			const uint32_t total = GW * A.lanes;
+			const uint32_t tk = (uint32_t)__hip_atomic_fetch_add(&ctl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (tk == total - 1) __hip_atomic_store(&ctl[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			ticket = tk;
+		}
+		__syncthreads();
+		grp = ticket;
+	}
+	const int32_t epoch = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + A.step + 1;
+	const uint32_t lane = grp / GW, j = grp % GW, c0 = j * G::W;
+	int32_t dig[E];
+	int64_t f;
+	colC_chunk<LN, RL, NP, W, false>(A, S, X, lane, c0, t, 0, dig, f);      // every row holds digits (N+)
+	const uint32_t r = t / hpr, h = t % hpr;
+	int64_t *xl = xc + (size_t)lane * GW * R;                               // [group][row]: carry of the row's last chunk
+	int32_t *fg = flags + (size_t)lane * GW;
+	// Ordering without cache maintenance: the carries, the flags and their
+	// reads are device-scope atomics (gfx950 `sc1` accesses, coherent across
+	// the XCDs' L2s).  Each storing thread waits for its store to complete
+	// before the barrier, the flag is stored after the barrier, and the
+	// carries are read after the barrier that follows the wait.  (Release and
+	// acquire orderings would write back and invalidate a whole L2 per access
+	// on this target.)
+	fl[t] = f;
+	if (h == hpr - 1) __hip_atomic_store(&xl[(size_t)j * R + r], f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__builtin_amdgcn_s_waitcnt(0);
+	__syncthreads();
+	const uint32_t jp = (j + GW - 1) % GW;
+	if (t == 0)
+	{
+		__hip_atomic_store(&fg[j], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint32_t spins = 0;
+		while (__hip_atomic_load(&fg[jp], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+		{
+			if (++spins > (1u << 22)) { __hip_atomic_store(&ctl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+			__builtin_amdgcn_s_sleep(2);
+		}
+	}
+	__syncthreads();
+	// incoming carry of chunk (r, h) = chunk g = r C/16 + j hpr + h
+	int64_t c;
+	if (h > 0) c = fl[t - 1];
+	else if (j > 0) c = __hip_atomic_load(&xl[(size_t)jp * R + r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	else
+	{
+		const int64_t top = __hip_atomic_load(&xl[(size_t)(GW - 1) * R + (R - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (r == 0) c = -top;
+		else
+		{
+			c = __hip_atomic_load(&xl[(size_t)(GW - 1) * R + (r - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (r == R / 2) c += top;
+		}
+	}
+	carry_in(dig, c, A.dv[lane]);
+	const uint32_t k = r * C + c0 + 16 * h;
+	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * n + k);
+#pragma unroll
+	for (int v = 0; v < 4; ++v) dst[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
+	// the digits of the next squaring in the column layout of pass A; S[0] is
+	// free (every read of S came before the barrier above)
+	int32_t *D = reinterpret_cast<int32_t *>(S[0]);
+#pragma unroll
+	for (int q = 0; q < E; ++q) D[(16 * h + q) * G::RP + r + (r >> 5)] = dig[q];
+	__syncthreads();
+	const uint32_t cw = t % G::W, tt = t / G::W;
+	colA_prime_all<LN, RL, NP, W>(D, X, A.res + (size_t)lane * n, A.tw, c0, cw, tt, (size_t)A.lanes * n, -1);
 }
 
 // normalize2 alone: materialises the pending carries into the digits
@@ -349,6 +471,12 @@ __global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, ui
 // sequence can be captured in a graph)
 __global__ void k_run_set(int32_t *run, int32_t i0) { run[0] = i0; run[1] = 0; }
 __global__ void k_run_add(int32_t *run, int32_t k) { run[1] += k; }
+// fused pass: advance the epoch base ctl[2] past a launch sequence
+__global__ void k_epoch_add(int32_t *ctl, int32_t k)
+{
+	__hip_atomic_store(&ctl[2], __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + k,
+	                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // ---------------------------------------------------------------------------
 // host sequencing
@@ -360,16 +488,47 @@ static int rows_log2(int ln) { return (ln <= 12) ? 4 : 8; }
 static bool tile_ok(int ln, uint32_t w) { return (ln <= 12) ? (w == 64) : (w == 16 || w == 32); }
 uint32_t plain_default_tile(int ln) { return (ln <= 12) ? 64 : ((ln >= 19) ? 32 : 16); }
 
+// parts of a squaring: 1 = column pass A, 2 = row pass B, 4 = column pass C
 template <int LN, int RL, int NP, int W, bool MINUS>
-static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st, int parts)
 {
 	using G = colgeo<LN, RL, W>;
 	constexpr uint32_t C = 1u << (LN - RL), R = 1u << RL;
 	const dim3 gcol(C / G::W, lanes);
-	if (A.split_a) hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, true>), dim3(C / G::W, lanes, NP), dim3(G::threads), 0, st, A, 0);
-	else hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, false>), gcol, dim3(G::threads), 0, st, A, 0);
-	hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
-	hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
+	if (parts & 1)
+	{
+		if (A.split_a) hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, true>), dim3(C / G::W, lanes, NP), dim3(G::threads), 0, st, A, 0);
+		else hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, false>), gcol, dim3(G::threads), 0, st, A, 0);
+	}
+	if (parts & 2) hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
+	if (parts & 4) hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
+	return hipGetLastError();
+}
+
+// Fused pass C + next pass A (N+, 256-row layout): flags per (lane, column
+// group), ctl = {error word, ticket counter, epoch base}, carries; whether the
+// grid fits the device at once (fits != null) or the launch.
+struct fused_ctl { int32_t *flags; int32_t *ctl; int64_t *xc; int use_ticket; };
+
+template <int LN, int NP, int W>
+static hipError_t fused_ln(const plain_args &A, uint32_t lanes, hipStream_t st, const fused_ctl &F, bool *fits)
+{
+	using G = colgeo<LN, 8, W>;
+	constexpr uint32_t C = 1u << (LN - 8);
+	if (fits)
+	{
+		int dev = 0, cus = 0, per = 0;
+		hipError_t h = hipGetDevice(&dev);
+		if (h == hipSuccess) h = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+		if (h == hipSuccess)
+			h = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_colCA<LN, 8, NP, W>),
+			                                                 (int)G::threads, 0);
+		if (h != hipSuccess) return h;
+		*fits = (uint64_t)per * (uint64_t)cus >= (uint64_t)(C / G::W) * lanes;
+		return hipSuccess;
+	}
+	hipLaunchKernelGGL((k_colCA<LN, 8, NP, W>), dim3((C / G::W) * lanes), dim3(G::threads), 0, st, A, F.flags, F.ctl, F.xc,
+	                   F.use_ticket);
 	return hipGetLastError();
 }
 
@@ -387,9 +546,9 @@ static hipError_t mul_ln(const plain_args &In, const plain_args &Out, uint32_t l
 }
 
 template <int LN, int RL, int W, bool MINUS>
-static hipError_t square_np(int np, const plain_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t square_np(int np, const plain_args &A, uint32_t lanes, hipStream_t st, int parts)
 {
-	return (np == 2) ? square_ln<LN, RL, 2, W, MINUS>(A, lanes, st) : square_ln<LN, RL, 3, W, MINUS>(A, lanes, st);
+	return (np == 2) ? square_ln<LN, RL, 2, W, MINUS>(A, lanes, st, parts) : square_ln<LN, RL, 3, W, MINUS>(A, lanes, st, parts);
 }
 
 template <int LN, int RL, int W, bool MINUS>
@@ -399,10 +558,11 @@ static hipError_t mul_np(int np, const plain_args &In, const plain_args &Out, ui
 }
 
 template <int LN, int RL, bool MINUS>
-static hipError_t square_w(int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t square_w(int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st, int parts)
 {
-	if constexpr (RL == 4) return square_np<LN, RL, 64, MINUS>(np, A, lanes, st);
-	else return (w == 32) ? square_np<LN, RL, 32, MINUS>(np, A, lanes, st) : square_np<LN, RL, 16, MINUS>(np, A, lanes, st);
+	if constexpr (RL == 4) return square_np<LN, RL, 64, MINUS>(np, A, lanes, st, parts);
+	else return (w == 32) ? square_np<LN, RL, 32, MINUS>(np, A, lanes, st, parts)
+	                      : square_np<LN, RL, 16, MINUS>(np, A, lanes, st, parts);
 }
 
 template <int LN, int RL, bool MINUS>
@@ -413,11 +573,23 @@ static hipError_t mul_w(int np, uint32_t w, const plain_args &In, const plain_ar
 }
 
 template <int LN, int RL>
-static hipError_t square_m(int np, uint32_t w, bool minus, const plain_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t square_m(int np, uint32_t w, bool minus, const plain_args &A, uint32_t lanes, hipStream_t st, int parts)
 {
 	if constexpr (LN >= 11)
-		if (minus) return square_w<LN, RL, true>(np, w, A, lanes, st);
-	return minus ? hipErrorInvalidValue : square_w<LN, RL, false>(np, w, A, lanes, st);
+		if (minus) return square_w<LN, RL, true>(np, w, A, lanes, st, parts);
+	return minus ? hipErrorInvalidValue : square_w<LN, RL, false>(np, w, A, lanes, st, parts);
+}
+
+template <int LN, int RL>
+static hipError_t fused_m(int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st, const fused_ctl &F,
+                          bool *fits)
+{
+	if constexpr (RL != 8) return hipErrorInvalidValue;
+	else
+	{
+		if (np == 2) return (w == 32) ? fused_ln<LN, 2, 32>(A, lanes, st, F, fits) : fused_ln<LN, 2, 16>(A, lanes, st, F, fits);
+		return (w == 32) ? fused_ln<LN, 3, 32>(A, lanes, st, F, fits) : fused_ln<LN, 3, 16>(A, lanes, st, F, fits);
+	}
 }
 
 template <int LN, int RL>
@@ -433,13 +605,26 @@ static hipError_t mul_m(int np, uint32_t w, bool minus, const plain_args &In, co
 	X(18, 8) X(19, 8) X(20, 8) X(21, 8) X(22, 8)
 
 // tln: log2 of the transform length (n digits for N+, 2n for N-)
-static hipError_t square_any(int tln, int np, uint32_t w, bool minus, const plain_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t square_any(int tln, int np, uint32_t w, bool minus, const plain_args &A, uint32_t lanes, hipStream_t st,
+                             int parts = 7)
 {
 	switch (tln)
 	{
-#define CT_SQ_CASE(L_, R_) case L_: return square_m<L_, R_>(np, w, minus, A, lanes, st);
+#define CT_SQ_CASE(L_, R_) case L_: return square_m<L_, R_>(np, w, minus, A, lanes, st, parts);
 	CT_PLAIN_SIZES(CT_SQ_CASE)
 #undef CT_SQ_CASE
+	default: return hipErrorInvalidValue;
+	}
+}
+
+static hipError_t fused_any(int tln, int np, uint32_t w, const plain_args &A, uint32_t lanes, hipStream_t st,
+                            const fused_ctl &F, bool *fits = nullptr)
+{
+	switch (tln)
+	{
+#define CT_FU_CASE(L_, R_) case L_: return fused_m<L_, R_>(np, w, A, lanes, st, F, fits);
+	CT_PLAIN_SIZES(CT_FU_CASE)
+#undef CT_FU_CASE
 	default: return hipErrorInvalidValue;
 	}
 }
@@ -472,6 +657,16 @@ int plain_alloc(ct_engine *e)
 	if (hipMalloc((void **)&e->d_carry, (size_t)e->lanes * (e->n / 16) * sizeof(int64_t)) != hipSuccess)
 		return fail(CT_ERR_NOMEM, "hipMalloc carries");
 	if (hipMalloc((void **)&e->d_run, 2 * sizeof(int32_t)) != hipSuccess) return fail(CT_ERR_NOMEM, "hipMalloc run");
+	// fused column pass: flags and carries per (lane, column group) of the
+	// narrowest tile, then {error word, ticket counter, epoch base}
+	const size_t groups = (size_t)e->lanes * ((e->tn >> e->rows_log2) / 16);
+	e->fused_bytes = (groups + 3) * sizeof(int32_t);
+	if (hipMalloc((void **)&e->d_flags, e->fused_bytes) != hipSuccess ||
+	    hipMalloc((void **)&e->d_xc, groups * (1u << e->rows_log2) * sizeof(int64_t)) != hipSuccess)
+		return fail(CT_ERR_NOMEM, "hipMalloc fused-pass buffers");
+	if (hipMemset(e->d_flags, 0, e->fused_bytes) != hipSuccess) return fail(CT_ERR_HIP, "hipMemset flags");
+	e->d_ctl = e->d_flags + groups;
+	e->fused_w = 0;
 	return CT_OK;
 }
 
@@ -484,8 +679,8 @@ static void drop_graphs(ct_engine *e)
 void plain_free(ct_engine *e)
 {
 	drop_graphs(e);
-	hipFree(e->d_res); hipFree(e->d_res2); hipFree(e->d_carry); hipFree(e->d_run);
-	e->d_res = e->d_res2 = nullptr; e->d_carry = nullptr; e->d_run = nullptr;
+	hipFree(e->d_res); hipFree(e->d_res2); hipFree(e->d_carry); hipFree(e->d_run); hipFree(e->d_flags); hipFree(e->d_xc);
+	e->d_res = e->d_res2 = nullptr; e->d_carry = nullptr; e->d_run = nullptr; e->d_flags = e->d_ctl = nullptr; e->d_xc = nullptr;
 }
 
 static plain_args make_args(ct_engine *e)
@@ -507,32 +702,78 @@ int plain_materialize(ct_engine *e)
 	                   e->reg(0), (const int64_t *)e->d_carry, (const divisor *)e->d_dv, e->n, e->lanes, e->digit_rows(), e->sign < 0 ? 1 : 0);
 	HIPCHK(hipGetLastError());
 	e->pending = false;
+	if (e->fused_check)
+	{   // the fused runs since the last read: did every carry exchange complete?
+		e->fused_check = false;
+		int32_t err = 0;
+		HIPCHK(hipMemcpyAsync(&err, e->d_ctl, sizeof(err), hipMemcpyDeviceToHost, e->st));
+		HIPCHK(hipStreamSynchronize(e->st));
+		if (err)
+		{
+			e->use_fused = false;
+			e->fused_w = 0;
+			return fail(CT_ERR_HIP, "fused column pass: a carry exchange between workgroups timed out");
+		}
+	}
 	return CT_OK;
 }
 
+// Launches `len` squarings of register 0, step indices relative to the run
+// counter: A B C per squaring, or with `fused` (len >= 2) A B, then (CA B) per
+// further squaring, then C, then the epoch base advances.  Either way the last
+// pass C leaves its carries pending.
+static hipError_t launch_seq(ct_engine *e, plain_args A, int len, bool fused, bool first_carry)
+{
+	const int tln = (int)e->tln;
+	hipError_t h = hipSuccess;
+	if (!fused || len < 2)
+	{
+		for (int j = 0; j < len && h == hipSuccess; ++j)
+		{
+			A.step = j;
+			A.carry_in = (j == 0 && !first_carry) ? nullptr : e->d_carry;
+			h = square_any(tln, e->np, e->col_w, e->sign < 0, A, e->lanes, e->st);
+		}
+		return h;
+	}
+	const fused_ctl F{e->d_flags, e->d_ctl, e->d_xc, e->fused_ticket ? 1 : 0};
+	A.step = 0;
+	A.carry_in = first_carry ? e->d_carry : nullptr;
+	h = square_any(tln, e->np, e->col_w, false, A, e->lanes, e->st, 3);
+	for (int j = 1; j < len && h == hipSuccess; ++j)
+	{
+		A.step = j - 1;                             // pass C belongs to squaring j - 1
+		h = fused_any(tln, e->np, e->col_w, A, e->lanes, e->st, F);
+		A.step = j;
+		if (h == hipSuccess) h = square_any(tln, e->np, e->col_w, false, A, e->lanes, e->st, 2);
+	}
+	A.step = len - 1;
+	if (h == hipSuccess) h = square_any(tln, e->np, e->col_w, false, A, e->lanes, e->st, 4);
+	if (h == hipSuccess)
+	{
+		hipLaunchKernelGGL(k_epoch_add, dim3(1), dim3(1), 0, e->st, e->d_ctl, (int32_t)len);
+		h = hipGetLastError();
+	}
+	return h;
+}
+
 // A run of squarings is launched as replays of captured graphs of up to
-// kGraphMax squarings (one graph per distinct length, first-carry state and
-// argument set; the last node advances run[1]) followed by eager launches.
+// kGraphMax squarings (one graph per distinct length, form, first-carry state
+// and argument set; the last node advances run[1]), or eagerly.
 static constexpr int kGraphMax = 128;
 static constexpr size_t kGraphCache = 16;
 
-static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
+static int graph_for(ct_engine *e, plain_args A, int len, bool fused, hipGraphExec_t *out)
 {
 	const bool fc = e->pending;
 	for (const auto &g : e->graphs)
 		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.base == e->base && g.split == e->split_a &&
-		    g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
+		    g.fused == fused && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
 		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
 	if (e->graphs.size() >= kGraphCache) drop_graphs(e);
 	hipGraph_t graph = nullptr;
 	HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
-	hipError_t h = hipSuccess;
-	for (int j = 0; j < len && h == hipSuccess; ++j)
-	{
-		A.step = j;
-		A.carry_in = (j == 0 && !fc) ? nullptr : e->d_carry;
-		h = square_any((int)e->tln, e->np, e->col_w, e->sign < 0, A, e->lanes, e->st);
-	}
+	hipError_t h = launch_seq(e, A, len, fused, fc);
 	if (h == hipSuccess)
 	{
 		hipLaunchKernelGGL(k_run_add, dim3(1), dim3(1), 0, e->st, e->d_run, (int32_t)len);
@@ -548,7 +789,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, hipGraphExec_t *out)
 	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	hipGraphDestroy(graph);
 	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
-	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
+	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, fused, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
 	*out = exec;
 	return CT_OK;
 }
@@ -561,27 +802,38 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 	A.run = e->d_run;
 	hipLaunchKernelGGL(k_run_set, dim3(1), dim3(1), 0, e->st, e->d_run, (int32_t)i_hi);
 	HIPCHK(hipGetLastError());
-	int s = 0, in_graphs = 0;
+	// the fused form: N+, 256-row layout, one exponent-bit index per run
+	const bool fused = e->use_fused && e->sign > 0 && e->rows_log2 == 8 && lane_i == nullptr && nsteps > 1;
+	if (fused && (e->fused_w != e->col_w || e->fused_np != e->np))
+	{   // a new kernel or grid shape: clear the flags, the words and the epoch
+		// base; tickets only when the grid does not fit the device at once
+		bool fits = false;
+		const fused_ctl F{e->d_flags, e->d_ctl, e->d_xc, 0};
+		HIPCHK(fused_any((int)e->tln, e->np, e->col_w, A, e->lanes, e->st, F, &fits));
+		e->fused_ticket = !fits || e->fused_force_ticket;
+		HIPCHK(hipMemsetAsync(e->d_flags, 0, e->fused_bytes, e->st));
+		e->fused_w = e->col_w;
+		e->fused_np = e->np;
+	}
 	if (e->use_graphs && nsteps > 1)
 	{
-		while (s < nsteps)
+		for (int s = 0; s < nsteps;)
 		{
 			const int len = std::min(kGraphMax, nsteps - s);
 			hipGraphExec_t exec = nullptr;
-			int rc = graph_for(e, A, len, &exec);
+			int rc = graph_for(e, A, len, fused && len > 1, &exec);
 			if (rc) return rc;
 			HIPCHK(hipGraphLaunch(exec, e->st));
 			e->pending = true;
-			s += len; in_graphs += len;
+			s += len;
 		}
 	}
-	for (; s < nsteps; ++s)
+	else
 	{
-		A.step = s - in_graphs;
-		A.carry_in = e->pending ? e->d_carry : nullptr;
-		HIPCHK(square_any((int)e->tln, e->np, e->col_w, e->sign < 0, A, e->lanes, e->st));
+		HIPCHK(launch_seq(e, A, nsteps, fused, e->pending));
 		e->pending = true;
 	}
+	if (fused) e->fused_check = true;
 	return CT_OK;
 }
 

@@ -71,11 +71,24 @@ int ct_engine_set_column_tile(ct_engine *e, uint32_t w);
  * the digits).  Results do not depend on it.  No counterpart in the reference. */
 int ct_engine_set_column_split(ct_engine *e, int on);
 
+/* Multipass engine (N+, n >= 2^13): fuse column pass C of each squaring with
+ * column pass A of the next (on = 1), the column workgroups exchanging their
+ * chunk carries through device memory within one launch, or keep three
+ * launches per squaring (on = 0).  on = 2 is on = 1 with the workgroups
+ * always ordered by ticket (otherwise only when the grid does not fit the
+ * device at once).  Used for runs without per-lane offsets; results do not
+ * depend on it.  A failed exchange is reported by the next
+ * read of register 0 and switches the fused form off.  The autotuner (csize
+ * 0) chooses it when it is faster.  No counterpart in the reference. */
+int ct_engine_set_fused(ct_engine *e, int on);
+
 /* What the autotuner chose and measured: the column tile (0 for resident
- * kernels), the split of column pass A, and ms per squaring for {resident,
- * multipass W=16, W=32, W=16 split, W=32 split} (-1 where not timed or when
- * the engine was not tuned). */
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, float *ms5);
+ * kernels), the split of column pass A, the fused column pass, and ms per
+ * squaring (-1 where not timed or when the engine was not tuned) in
+ * ms[CT_TUNE_TIMINGS]: {resident, multipass W=16, W=32, W=16 split,
+ * W=32 split, W=16 fused, W=32 fused}.  Any output may be null. */
+#define CT_TUNE_TIMINGS 7
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, float *ms);
 int ct_engine_destroy(ct_engine *e);
 
 /* transform::set_b (transform.h:472-486): b[lanes], 32 < b < 2^31. */

@@ -28,16 +28,23 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     bs = [rng.randrange(1 << 17, 1 << 20) for _ in range(lanes)]
     print(f"\n[case] tuned ln={ln} lanes={lanes}", flush=True)
     tuned = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
-    w, split, ms = tuned.tuning()
+    w, split, fused, ms = tuned.tuning()
+    assert len(ms) == ct.TUNE_TIMINGS
     timed = [m for m in ms if m > 0]
     assert timed, ms
+    # the fused column pass is timed exactly for N+ with 256-row columns
+    assert (ms[5] > 0) == (ln >= 13 and ms[1] > 0)
     # the chosen configuration is the fastest one timed
     best = min(timed)
     if tuned.kind == ct.KIND_RESIDENT:
-        assert ms[0] == best and w == 0 and not split
+        assert ms[0] == best and w == 0 and not split and not fused
     else:
         assert w in (16, 32, 64)
-        idx = (1 if w in (16, 64) else 2) + (2 if split else 0)
+        if fused:
+            assert not split
+            idx = 5 if w == 16 else 6
+        else:
+            idx = (1 if w in (16, 64) else 2) + (2 if split else 0)
         assert ms[idx] == best
     ref = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=16)
     a = _run(tuned, bs, n, 40, 1)
@@ -60,7 +67,7 @@ def test_column_tiles_give_identical_digits(ln):
             t = ct.Transform(ln, 1, ct.KIND_PLAIN)
             t.set_column_tile(w)
             t.set_column_split(split)
-            assert t.tuning()[:2] == (w, split)
+            assert t.tuning()[:3] == (w, split, False)
             out.append(_run(t, bs, n, 30, 2))
     for o in out[1:]:
         assert np.array_equal(out[0], o)
