@@ -59,7 +59,10 @@ struct colgeo
 	static constexpr uint32_t Tc = R / 16;                 // threads per column
 	static constexpr uint32_t W = W_;                      // columns per workgroup
 	static constexpr uint32_t threads = W * Tc;            // = chunks per workgroup (R W / 16)
-	static constexpr uint32_t RP = R + (R >> 5);           // padded column stride in LDS
+	// padded column stride in LDS: a column's rows r sit at r + r/32 (the
+	// layout of the in-column transposes); the odd stride puts the W columns
+	// that a wavefront touches at once in distinct banks
+	static constexpr uint32_t RP = R + (R >> 5) + 1;
 	static constexpr int passes = RL / 4;
 };
 
