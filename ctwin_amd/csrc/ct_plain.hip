@@ -347,8 +347,8 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 // Column pass C of one squaring fused with column pass A of the next (N+):
 // the inverse column levels, CRT and normalize1 as in k_colC, then the chunk
 // carries cross to the next column group (normalize2 needs the carry of the
-// chunk before each chunk, kernel.cl:1053-1081), then the digits go to HBM and,
-// through LDS, straight into the forward column levels of the next squaring.
+// chunk before each chunk, kernel.cl:1053-1081), then the carried digits go
+// through LDS straight into the forward column levels of the next squaring.
 //
 // Exchange.  Group j publishes the carries of its last chunk per row in xc,
 // then raises flags[j] to this launch's epoch and waits until flags[j - 1]
@@ -434,11 +434,8 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 		}
 	}
 	carry_in(dig, c, A.dv[lane]);
-	const uint32_t k = r * C + c0 + 16 * h;
-	int4 *dst = reinterpret_cast<int4 *>(A.x + (size_t)lane * n + k);
-#pragma unroll
-	for (int v = 0; v < 4; ++v) dst[v] = make_int4(dig[4 * v], dig[4 * v + 1], dig[4 * v + 2], dig[4 * v + 3]);
-	// the digits of the next squaring in the column layout of pass A; S[0] is
+	// the digits of the next squaring in the column layout of pass A (they do
+	// not go to HBM: only the run's last pass C writes the register); S[0] is
 	// free (every read of S came before the barrier above)
 	int32_t *D = reinterpret_cast<int32_t *>(S[0]);
 #pragma unroll
