@@ -2,7 +2,8 @@
 10,000 PRP-loop squarings on the plain engine.  Checked against GMP residues
 (RES64 and x mod three primes, every 1000 squarings, written by
 tests/golden/gen_large_residues.c) and, over all 10,000 squarings, against the
-engine's own three-prime path (different arithmetic, same carried digits)."""
+engine's own three-prime path (different arithmetic, same carried digits) and
+its fused column pass (pass C and the next pass A in one launch)."""
 import json
 import os
 
@@ -52,11 +53,12 @@ def test_large_candidate_10000_squarings():
     Ew = top << s
     print(f"\n[case] large n=2^{ln} b={b} steps={steps} checkpoints={sorted(checks)}", flush=True)
     engines = []
-    for np_mode in (0, 3):
+    for np_mode, fused in ((0, 0), (3, 0), (0, 1)):
         t = ct.Transform(ln, 1, ct.KIND_PLAIN)
         t.set_b([b]); t.set_primes(np_mode); t.set_exponents([Ew]); t.set(1)
+        t.set_fused(fused)
         engines.append(t)
-    assert engines[0].primes == 2 and engines[1].primes == 3
+    assert engines[0].primes == 2 and engines[1].primes == 3 and engines[2].primes == 2
     done = 0
     for chunk_end in range(1000, steps + 1, 1000):
         states = []
@@ -64,6 +66,7 @@ def test_large_candidate_10000_squarings():
             t.run(i0 - done, i0 - chunk_end + 1)
             states.append(t.read_state(0))
         assert np.array_equal(states[0], states[1]), f"2-prime and 3-prime digits differ at t={chunk_end}"
+        assert np.array_equal(states[0], states[2]), f"fused column pass digits differ at t={chunk_end}"
         done = chunk_end
         if chunk_end in checks:
             canon = engines[0].get_int(0)[0]
