@@ -111,20 +111,31 @@ struct ptw_lazy                           // the same roots read where they are 
 };
 struct mtw { tw_entry r, K; };           // level-1 merge constants (inverse table entries 0, 1)
 
+// Block index of thread tt in pass L.  When a block spans a multiple of 64
+// threads, every lane of a wavefront has the same block, and readfirstlane
+// makes that visible to the compiler: the roots are then read with scalar
+// loads into SGPRs.  (tt is the thread index in every caller that reaches a
+// pass with mm >= 64: the resident kernels and the row pass.)
+template <int LNL, int L>
+__device__ __forceinline__ uint32_t pass_block(uint32_t tt)
+{
+	constexpr uint32_t mm = (geo<LNL>::n >> (4 * L)) / 16;
+	if constexpr (L == 0) return 0u;
+	else if constexpr (mm >= 64 && mm % 64 == 0) return __builtin_amdgcn_readfirstlane(tt / mm);
+	else return tt / mm;
+}
+
 template <int LNL, int L>
 __device__ __forceinline__ ptw_lazy lazy_ptw(const tw_entry *tw, uint32_t base, uint32_t tt)
 {
-	constexpr uint32_t mm = (geo<LNL>::n >> (4 * L)) / 16;
-	ptw_lazy T; T.tw = tw; T.sj = (base << (4 * L)) + ((L == 0) ? 0u : tt / mm);
+	ptw_lazy T; T.tw = tw; T.sj = (base << (4 * L)) + pass_block<LNL, L>(tt);
 	return T;
 }
 
 template <int LNL, int L>
 __device__ __forceinline__ void load_ptw(ptw &T, const tw_entry *__restrict__ tw, uint32_t base, uint32_t tt)
 {
-	constexpr uint32_t mm = (geo<LNL>::n >> (4 * L)) / 16;
-	const uint32_t j = (L == 0) ? 0u : tt / mm;
-	const uint32_t sj = (base << (4 * L)) + j;
+	const uint32_t sj = (base << (4 * L)) + pass_block<LNL, L>(tt);
 #pragma unroll
 	for (int q = 0; q < 4; ++q)
 	{
