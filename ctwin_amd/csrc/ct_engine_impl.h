@@ -55,7 +55,7 @@ struct ct_engine
 	uint32_t base = 2;                // PRP base: a set exponent bit multiplies by it (2 or 3)
 	// captured squaring sequences of the multipass engine (hipGraph), keyed
 	// by what the captured kernel arguments depend on
-	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool split, fused, first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
+	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool split, fused, ticket, first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
 	std::vector<graph_entry> graphs;
 	bool use_graphs = true;
 	int32_t *d_run = nullptr;         // {top bit index of the run, squarings done by replayed graphs}

@@ -91,6 +91,41 @@ __device__ __forceinline__ void xchg(V (&u)[E], V *lds, uint32_t tt)
 	for (int q = 0; q < E; ++q) u[q] = rd[loff<LNL, TO>(q)];
 }
 
+// Layout L when tt is the workgroup's flat thread index: the 64 threads of a
+// wavefront own 1024 consecutive elements -- a contiguous padded LDS range
+// of their own -- in the final layout and whenever a block spans at most one
+// wavefront (mm <= 64).
+template <int LNL, int L>
+__device__ __forceinline__ constexpr bool wave_local()
+{
+	if constexpr (L == geo<LNL>::a) return true;
+	else return ((geo<LNL>::n >> (4 * L)) / 16) <= 64;
+}
+
+// Transpose for flat thread indices: when both layouts keep every wavefront's
+// elements in its own LDS range, and the previous transpose already left each
+// wavefront alone in its range, no other wavefront reads or writes it, so the
+// workgroup barriers reduce to ordering within the wavefront (its LDS
+// operations execute in order; wave_barrier keeps the compiler from moving
+// them across).
+template <int LNL, int FROM, int TO, typename V>
+__device__ __forceinline__ void xchg_flat(V (&u)[E], V *lds, uint32_t tt)
+{
+	if constexpr (wave_local<LNL, FROM>() && wave_local<LNL, TO>())
+	{
+		V *wr = lds + lbase<LNL, FROM>(tt);
+		V *rd = lds + lbase<LNL, TO>(tt);
+		__builtin_amdgcn_wave_barrier();
+#pragma unroll
+		for (int q = 0; q < E; ++q) wr[loff<LNL, FROM>(q)] = u[q];
+		__builtin_amdgcn_wave_barrier();
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = rd[loff<LNL, TO>(q)];
+		__builtin_amdgcn_wave_barrier();
+	}
+	else xchg<LNL, FROM, TO>(u, lds, tt);
+}
+
 // ---- twiddles of one pass, loaded one pass ahead of their use -------------
 //
 // A radix-16 pass over levels 4L+1..4L+4 of an instance needs 1 + 2 + 4 + 8
@@ -351,7 +386,7 @@ __device__ __forceinline__ void fwd_chain(uint32_t (&u)[E], uint32_t *lds, const
 			fwd_pass<P, PHI6 && L == 0>(u, cur);
 		}
 		else fwd_pass<P, PHI6 && L == 0>(u, lazy_ptw<LNL, L>(twf, base, tt));
-		xchg<LNL, L, L + 1>(u, lds, tt);
+		xchg_flat<LNL, L, L + 1>(u, lds, tt);
 		if constexpr (!last) fwd_chain<LNL, P, L + 1, PHI6, PF>(u, lds, twf, twi, base, tt, nxt, F);
 	}
 }
@@ -376,7 +411,7 @@ __device__ __forceinline__ void inv_chain(uint32_t (&u)[E], uint32_t *lds, const
 	{
 		ptw nxt;
 		if constexpr (PF && L > 0) load_ptw<LNL, L - 1>(nxt, twi, base, tt);
-		xchg<LNL, L + 1, L>(u, lds, tt);
+		xchg_flat<LNL, L + 1, L>(u, lds, tt);
 		if constexpr (PF) inv_pass<P, PHI6 && L == 0>(u, cur, M);
 		else inv_pass<P, PHI6 && L == 0>(u, lazy_ptw<LNL, L>(twi, base, tt), M);
 		if constexpr (L > 0) inv_chain<LNL, P, L - 1, PHI6, PF>(u, lds, twi, base, tt, nxt, M);

@@ -388,7 +388,9 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 		__syncthreads();
 		grp = ticket;
 	}
-	const int32_t epoch = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + A.step + 1;
+	// unsigned: the epoch wraps after 2^32 launches (flags are only compared for equality)
+	const int32_t epoch = (int32_t)((uint32_t)__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+	                                (uint32_t)A.step + 1u);
 	const uint32_t lane = grp / GW, j = grp % GW, c0 = j * G::W;
 	int32_t dig[E];
 	int64_t f;
@@ -474,8 +476,8 @@ __global__ void k_run_add(int32_t *run, int32_t k) { run[1] += k; }
 // fused pass: advance the epoch base ctl[2] past a launch sequence
 __global__ void k_epoch_add(int32_t *ctl, int32_t k)
 {
-	__hip_atomic_store(&ctl[2], __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + k,
-	                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	const uint32_t e = (uint32_t)__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + (uint32_t)k;
+	__hip_atomic_store(&ctl[2], (int32_t)e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -768,7 +770,8 @@ static int graph_for(ct_engine *e, plain_args A, int len, bool fused, hipGraphEx
 	const bool fc = e->pending;
 	for (const auto &g : e->graphs)
 		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.base == e->base && g.split == e->split_a &&
-		    g.fused == fused && g.first_carry == fc && g.bits == A.bits && g.mask == A.mask &&
+		    g.fused == fused && g.ticket == (fused && e->fused_ticket) && g.first_carry == fc && g.bits == A.bits &&
+		    g.mask == A.mask &&
 		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
 	if (e->graphs.size() >= kGraphCache) drop_graphs(e);
 	hipGraph_t graph = nullptr;
@@ -789,7 +792,8 @@ static int graph_for(ct_engine *e, plain_args A, int len, bool fused, hipGraphEx
 	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	hipGraphDestroy(graph);
 	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
-	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, fused, fc, A.bits, A.mask, A.lane_i, A.bit_words, exec});
+	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, fused, fused && e->fused_ticket, fc, A.bits, A.mask,
+	                     A.lane_i, A.bit_words, exec});
 	*out = exec;
 	return CT_OK;
 }
