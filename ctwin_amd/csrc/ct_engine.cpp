@@ -97,29 +97,34 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 // multipass launch when there are few candidates), the multipass column tile
 // (16 or 32 columns per workgroup), column pass A combined or split by prime
 // and, for N+ with 256-row columns, pass C fused with the next pass A -- and
-// keeps the fastest.  Results are
-// cached per (device, log2 n, lanes) for the life of the process.
+// keeps the fastest.  Results are cached per (device, log2 n, lanes, kind,
+// sign, prime count) for the life of the process.  An engine is tuned at
+// creation for three primes (b is not known yet); when set_b or set_primes
+// later moves a tuned multipass engine to two primes (or back), its column
+// tile, split and fused form are tuned again for that prime count.
 
 namespace {
 
-struct tune_key { int device; uint32_t ln, lanes; int kind, sign;
+struct tune_key { int device; uint32_t ln, lanes; int kind, sign, np;
 	bool operator<(const tune_key &o) const
-	{ return std::tie(device, ln, lanes, kind, sign) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign); } };
+	{ return std::tie(device, ln, lanes, kind, sign, np) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign, o.np); } };
 struct tune_val { int kind; uint32_t col_w; bool split, fused; float ms[CT_TUNE_TIMINGS]; };
 std::mutex g_tune_mu;
 std::map<tune_key, tune_val> g_tune;
 
 constexpr int kTuneSteps = 24, kTuneReps = 2;
 
-// ms per squaring of engine e (3 primes, b = 2^30 + 9699690 i as in
-// transform.h:391 but below 2^31), or a negative value on failure: the
-// fastest of kTuneReps timed runs of kTuneSteps squarings after a warm-up run
-// of the same length (which also captures the graph of that length).
-float time_squarings(ct_engine *e)
+// ms per squaring of engine e with np primes, or a negative value on failure:
+// the fastest of kTuneReps timed runs of kTuneSteps squarings after a warm-up
+// run of the same length (which also captures the graph of that length).
+// Three primes: b = 2^30 + 9699690 i as in transform.h:391 (but below 2^31);
+// two primes: b = 65537 + 2 i, for which 4 n b^2 < p1 p2 / 2 up to n = 2^22.
+float time_squarings(ct_engine *e, int np)
 {
 	std::vector<uint32_t> b(e->lanes);
-	for (uint32_t i = 0; i < e->lanes; ++i) b[i] = (1u << 30) + 9699690u * (i % 100);
+	for (uint32_t i = 0; i < e->lanes; ++i) b[i] = (np == 3) ? (1u << 30) + 9699690u * (i % 100) : 65537u + 2u * (i % 100);
 	if (ct_engine_set_b(e, b.data()) || ct_engine_set(e, 3)) return -1;
+	if (e->np != np) return -1;
 	if (ct::steps(e, nullptr, 0, 0, kTuneSteps, nullptr)) return -1;
 	float best = 1e30f;
 	for (int r = 0; r < kTuneReps; ++r)
@@ -134,9 +139,9 @@ float time_squarings(ct_engine *e)
 
 }   // namespace
 
-static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tune_val *out)
+static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, int np, tune_val *out)
 {
-	const tune_key key{device, ln, lanes, kind, sign};
+	const tune_key key{device, ln, lanes, kind, sign, np};
 	const uint32_t tln = ln + (sign < 0 ? 1 : 0);
 	{
 		std::lock_guard<std::mutex> g(g_tune_mu);
@@ -152,7 +157,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 		ct_engine *e = nullptr;
 		int rc = create_impl(device, ln, lanes, CT_KIND_RESIDENT, 0, false, sign, &e);
 		if (rc) return rc;
-		v.ms[0] = time_squarings(e);
+		v.ms[0] = time_squarings(e, np);
 		ct_engine_destroy(e);
 		if (v.ms[0] < 0) return fail(CT_ERR_HIP, "autotuner: timing the resident kernels failed");
 		if (v.ms[0] < best) { best = v.ms[0]; v.kind = CT_KIND_RESIDENT; v.col_w = 0; }
@@ -168,7 +173,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 			{
 				if (!plain_tile_ok((int)tln, tiles[t])) continue;
 				e->col_w = tiles[t]; e->split_a = (sp != 0);
-				const float ms = time_squarings(e);
+				const float ms = time_squarings(e, np);
 				if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
 				v.ms[1 + t + 2 * sp] = ms;
 				if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = (sp != 0); v.fused = false; }
@@ -178,7 +183,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 			{
 				if (!plain_tile_ok((int)tln, tiles[t])) continue;
 				e->col_w = tiles[t]; e->split_a = false; e->use_fused = true;
-				float ms = time_squarings(e);
+				float ms = time_squarings(e, np);
 				// a failed exchange surfaces at the next read of register 0
 				if (ms >= 0 && plain_materialize(e) != CT_OK) ms = -1;
 				e->use_fused = false;
@@ -188,7 +193,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, tun
 			}
 		if (!plain_tile_ok((int)tln, 16) && !plain_tile_ok((int)tln, 32))
 		{   // small n: one tile only
-			const float ms = time_squarings(e);
+			const float ms = time_squarings(e, np);
 			if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
 			v.ms[1] = ms;
 			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = plain_default_tile((int)tln); }
@@ -220,13 +225,15 @@ int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t cs
 		return fail(CT_ERR_UNSUPPORTED, "multipass engine needs log2_n >= 10");
 	if (csize == 16) return create_impl(device, log2_n, lanes, kind, 0, false, sign, out);
 	tune_val v;
-	int rc = tune(device, log2_n, lanes, kind, sign, &v);
+	int rc = tune(device, log2_n, lanes, kind, sign, 3, &v);
 	if (rc) return rc;
 	rc = create_impl(device, log2_n, lanes, v.kind, v.col_w, v.split, sign, out);
 	if (rc == CT_OK)
 	{
 		std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
 		(*out)->use_fused = v.fused;
+		(*out)->tuned = true;
+		(*out)->tuned_np = 3;
 	}
 	return rc;
 }
@@ -322,6 +329,21 @@ int ct_engine_moduli(uint32_t *p)
 // ---------------------------------------------------------------------------
 // parameters and registers
 
+// A tuned multipass engine whose prime count changed: tune its column tile,
+// pass-A split and fused form for the new count (the engine kind stays).
+static int retune_for_primes(ct_engine *e)
+{
+	if (!e->tuned || e->kind != CT_KIND_PLAIN || e->np == e->tuned_np) return CT_OK;
+	tune_val v;
+	const int rc = tune(e->device, e->ln, e->lanes, CT_KIND_PLAIN, e->sign, e->np, &v);
+	if (rc) return rc;
+	HIPCHK(hipSetDevice(e->device));
+	e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->fused_w = 0;
+	for (int i = 1; i < CT_TUNE_TIMINGS; ++i) e->tune_ms[i] = v.ms[i];
+	e->tuned_np = e->np;
+	return CT_OK;
+}
+
 int ct_engine_set_b(ct_engine *e, const uint32_t *b)
 {
 	if (!e || !b) return fail(CT_ERR_ARG, "null argument");
@@ -339,7 +361,7 @@ int ct_engine_set_b(ct_engine *e, const uint32_t *b)
 	e->b_max = *std::max_element(e->b.begin(), e->b.end());
 	e->state_bound = 1;     // registers must be (re)initialised for the new modulus
 	choose_primes(e);
-	return CT_OK;
+	return retune_for_primes(e);
 }
 
 int ct_engine_set(ct_engine *e, uint32_t a)
@@ -373,7 +395,7 @@ int ct_engine_set_primes(ct_engine *e, int np)
 	if (!e || (np != 0 && np != 3)) return fail(CT_ERR_ARG, "np must be 0 (automatic) or 3");
 	e->force3 = (np == 3);
 	choose_primes(e);
-	return CT_OK;
+	return retune_for_primes(e);
 }
 
 int ct_engine_primes(const ct_engine *e, int *np)

@@ -47,6 +47,8 @@ struct ct_engine
 	// autotuner timings (ms per squaring): resident, multipass W=16, W=32, and
 	// the same two tiles with column pass A split by prime
 	float tune_ms[CT_TUNE_TIMINGS] = {-1, -1, -1, -1, -1, -1, -1};
+	bool tuned = false;               // created with csize 0 (autotuned)
+	int tuned_np = 0;                 // prime count the multipass parameters were tuned for
 	bool pending = false;             // register 0 still owes the chunk carries (plain engine)
 	int np = 3;                       // primes per squaring: 2 when 3 n D^2 < p1 p2 / 2
 	uint64_t b_max = 0;               // largest b over the lanes

@@ -50,9 +50,16 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     a = _run(tuned, bs, n, 40, 1)
     b = _run(ref, bs, n, 40, 1)
     assert np.array_equal(a, b)
+    # set_b moved the engine to tuned.primes primes; a multipass engine is
+    # tuned again for that count and runs the fastest configuration timed
+    w2, split2, fused2, ms2 = tuned.tuning()
+    if tuned.kind == ct.KIND_PLAIN:
+        timed2 = [m for m in ms2[1:] if m > 0]
+        idx2 = (5 if w2 == 16 else 6) if fused2 else (1 if w2 in (16, 64) else 2) + (2 if split2 else 0)
+        assert ms2[idx2] == min(timed2)
     # second creation hits the cache: same choice, same timings
     again = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
-    assert again.kind == tuned.kind and again.tuning() == tuned.tuning()
+    assert again.kind == tuned.kind and again.tuning() == (w, split, fused, ms)
 
 
 @pytest.mark.parametrize("ln", [13, 16, 20])
