@@ -47,9 +47,14 @@ typedef struct ct_engine ct_engine;
  * n <= 2^14, else multipass).  csize = 0 runs the autotuner first, as the
  * reference does for csize 0 (transform.h:383-419, which times CSIZE 8..64):
  * it times squarings under each compiled configuration -- resident kernels
- * vs the multipass engine when kind is AUTO and n <= 2^14, and the multipass
- * column tile -- keeps the fastest, and caches the choice per (device, n,
- * lanes, kind) for the process.  Allocates 3 digit registers per lane. */
+ * vs the multipass engine when kind is AUTO and n <= 2^14, and for the
+ * multipass engine the column tile, the split of column pass A and the fused
+ * column pass -- keeps the fastest, and caches the choice per (device, n,
+ * lanes, kind, sign, prime count) for the process.  The engine is tuned for
+ * three primes at creation; when ct_engine_set_b or ct_engine_set_primes
+ * moves a tuned multipass engine to another prime count, its multipass
+ * parameters are tuned again for that count (the engine kind stays).
+ * Allocates 3 digit registers per lane. */
 int ct_engine_create(int device, uint32_t log2_n, uint32_t lanes, uint32_t csize, int kind, ct_engine **out);
 
 /* ct_engine_create for either twin member: sign = +1 gives the engine above
