@@ -240,6 +240,41 @@ __device__ __forceinline__ uint32_t fold_minus(const uint32_t *S, uint32_t col_b
 	return red1p(red1p(at(r) + at(r + R / 4), p) + red1p(c2 + c2, p), p);
 }
 
+// Inverse column levels of prime P for 256-row columns, in two halves around
+// one transpose that every prime shares: pass 1 in registers, then (after
+// the caller's write of layout 1, a barrier and the read of layout 0) pass 0
+// with the level-1 merge.
+template <int RL, int P>
+__device__ __forceinline__ void col_inv_hi(uint32_t (&u)[E], const tw_entry *twi, uint32_t tt)
+{
+	ptw c;
+	load_ptw<RL, 1>(c, twi, 1u, tt);
+	inv_pass<P, false>(u, c, load_mtw(twi));
+}
+template <int RL, int P>
+__device__ __forceinline__ void col_inv_lo(uint32_t (&u)[E], const tw_entry *twi, uint32_t tt)
+{
+	ptw c;
+	load_ptw<RL, 0>(c, twi, 1u, tt);
+	inv_pass<P, true>(u, c, load_mtw(twi));
+}
+// Forward column levels of prime P for 256-row columns around one shared
+// transpose: pass 0 (with the Phi6 split) from layout 0, then pass 1.
+template <int RL, int P>
+__device__ __forceinline__ void col_fwd_lo(uint32_t (&u)[E], const tw_entry *twf, uint32_t tt)
+{
+	ptw c;
+	load_ptw<RL, 0>(c, twf, 1u, tt);
+	fwd_pass<P, true>(u, c);
+}
+template <int RL, int P>
+__device__ __forceinline__ void col_fwd_hi(uint32_t (&u)[E], const tw_entry *twf, uint32_t tt)
+{
+	ptw c;
+	load_ptw<RL, 1>(c, twf, 1u, tt);
+	fwd_pass<P, false>(u, c);
+}
+
 // The inverse column levels, the CRT and normalize1 of column pass C for the
 // chunk role of thread t (row r, chunk h of the group's W columns): the
 // chunk's digits and outgoing carry.  Returns false for threads whose row
@@ -265,20 +300,61 @@ __device__ __forceinline__ bool colC_chunk(const plain_args &A, uint32_t (*S)[co
 #pragma unroll
 		for (int q = 0; q < E; ++q) uu[P][q] = in[(size_t)elem<RL, La>(tt, q) * C];
 	}
-#pragma unroll
-	for (int P = 0; P < NP; ++P)
+	if constexpr (G::passes == 2)
 	{
-		const tw_entry *twi = A.tw + (size_t)(P * 2 + 1) * n;
-		uint32_t *Xc = X + cw * G::RP;
-		ptw c;
-		load_ptw<RL, La>(c, twi, 1u, tt);
-		const mtw M = load_mtw(twi);
-		if (P == 0) col_inv<RL, 0, La>(uu[0], Xc, twi, tt, c, M);
-		else if (P == 1) col_inv<RL, 1, La>(uu[1], Xc, twi, tt, c, M);
-		else col_inv<RL, 2, La>(uu[NP - 1], Xc, twi, tt, c, M);
-		uint32_t *sw = S[P] + cw * G::RP + lbase<RL, 0>(tt);
+		// 256-row columns: every prime's pass 1, one transpose through each
+		// prime's own S[P] (a single barrier for all of them), pass 0 and the
+		// merge, the result written back in place (each thread rewrites exactly
+		// the words it read)
+		const tw_entry *twi[NP];
 #pragma unroll
-		for (int q = 0; q < E; ++q) sw[loff<RL, 0>(q)] = uu[P][q];
+		for (int P = 0; P < NP; ++P) twi[P] = A.tw + (size_t)(P * 2 + 1) * n;
+		col_inv_hi<RL, 0>(uu[0], twi[0], tt);
+		col_inv_hi<RL, 1>(uu[1], twi[1], tt);
+		if constexpr (NP == 3) col_inv_hi<RL, 2>(uu[NP - 1], twi[NP - 1], tt);
+#pragma unroll
+		for (int P = 0; P < NP; ++P)
+		{
+			uint32_t *sw = S[P] + cw * G::RP + lbase<RL, 1>(tt);
+#pragma unroll
+			for (int q = 0; q < E; ++q) sw[loff<RL, 1>(q)] = uu[P][q];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int P = 0; P < NP; ++P)
+		{
+			const uint32_t *sr = S[P] + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+			for (int q = 0; q < E; ++q) uu[P][q] = sr[loff<RL, 0>(q)];
+		}
+		col_inv_lo<RL, 0>(uu[0], twi[0], tt);
+		col_inv_lo<RL, 1>(uu[1], twi[1], tt);
+		if constexpr (NP == 3) col_inv_lo<RL, 2>(uu[NP - 1], twi[NP - 1], tt);
+#pragma unroll
+		for (int P = 0; P < NP; ++P)
+		{
+			uint32_t *sw = S[P] + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+			for (int q = 0; q < E; ++q) sw[loff<RL, 0>(q)] = uu[P][q];
+		}
+	}
+	else
+	{
+#pragma unroll
+		for (int P = 0; P < NP; ++P)
+		{
+			const tw_entry *twi = A.tw + (size_t)(P * 2 + 1) * n;
+			uint32_t *Xc = X + cw * G::RP;
+			ptw c;
+			load_ptw<RL, La>(c, twi, 1u, tt);
+			const mtw M = load_mtw(twi);
+			if (P == 0) col_inv<RL, 0, La>(uu[0], Xc, twi, tt, c, M);
+			else if (P == 1) col_inv<RL, 1, La>(uu[1], Xc, twi, tt, c, M);
+			else col_inv<RL, 2, La>(uu[NP - 1], Xc, twi, tt, c, M);
+			uint32_t *sw = S[P] + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+			for (int q = 0; q < E; ++q) sw[loff<RL, 0>(q)] = uu[P][q];
+		}
 	}
 	__syncthreads();
 	// chunk role (threads whose row holds digits)
@@ -443,8 +519,57 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 #pragma unroll
 	for (int q = 0; q < E; ++q) D[(16 * h + q) * G::RP + r + (r >> 5)] = dig[q];
 	__syncthreads();
+	// forward column levels of every prime around one shared transpose: the
+	// digits are read once, prime P transposes through bufs[P] (X, then the
+	// S[P] that the CRT has finished reading; S[0] holds the digits)
 	const uint32_t cw = t % G::W, tt = t / G::W;
-	colA_prime_all<LN, RL, NP, W>(D, X, A.res + (size_t)lane * n, A.tw, c0, cw, tt, (size_t)A.lanes * n, -1);
+	int32_t d[E];
+	{
+		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) d[q] = dr[loff<RL, 0>(q)];
+	}
+	uint32_t *bufs[NP];
+	bufs[0] = X;
+#pragma unroll
+	for (int P = 1; P < NP; ++P) bufs[P] = S[P];
+	const tw_entry *twf[NP];
+#pragma unroll
+	for (int P = 0; P < NP; ++P) twf[P] = A.tw + (size_t)(P * 2) * n;
+	uint32_t u[NP][E];
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[P][q] = from_signed(d[q], kP[P]);
+	col_fwd_lo<RL, 0>(u[0], twf[0], tt);
+	col_fwd_lo<RL, 1>(u[1], twf[1], tt);
+	if constexpr (NP == 3) col_fwd_lo<RL, 2>(u[NP - 1], twf[NP - 1], tt);
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+	{
+		uint32_t *bw = bufs[P] + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) bw[loff<RL, 0>(q)] = u[P][q];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+	{
+		const uint32_t *br = bufs[P] + cw * G::RP + lbase<RL, 1>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[P][q] = br[loff<RL, 1>(q)];
+	}
+	col_fwd_hi<RL, 0>(u[0], twf[0], tt);
+	col_fwd_hi<RL, 1>(u[1], twf[1], tt);
+	if constexpr (NP == 3) col_fwd_hi<RL, 2>(u[NP - 1], twf[NP - 1], tt);
+	uint32_t *res_lane = A.res + (size_t)lane * n;
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+	{
+		uint32_t *out = res_lane + P * (size_t)A.lanes * n + c0 + cw;
+#pragma unroll
+		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, 1>(tt, q) * C] = u[P][q];
+	}
 }
 
 // normalize2 alone: materialises the pending carries into the digits
