@@ -132,6 +132,63 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 	}
 }
 
+// Forward column levels of every prime for 256-row columns around one
+// transpose that all primes share (one barrier): the digits (layout 0 of D)
+// are read once, prime P goes through pass 0, is transposed through its own
+// free LDS plane bufs[P], then pass 1, and its residues are stored.  D_REUSED:
+// one of the planes is D itself, and a barrier separates every thread's digit
+// reads from the plane writes.
+template <int LN, int RL, int NP, int W, bool D_REUSED>
+__device__ __forceinline__ void colA_all_shared(const int32_t *D, uint32_t *const (&bufs)[NP], uint32_t *res_lane,
+                                                const tw_entry *tw, uint32_t c0, uint32_t cw, uint32_t tt, size_t plane)
+{
+	using G = colgeo<LN, RL, W>;
+	constexpr uint32_t n = 1u << LN, C = n >> RL;
+	int32_t d[E];
+	{
+		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) d[q] = dr[loff<RL, 0>(q)];
+	}
+	const tw_entry *twf[NP];
+#pragma unroll
+	for (int P = 0; P < NP; ++P) twf[P] = tw + (size_t)(P * 2) * n;
+	uint32_t u[NP][E];
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[P][q] = from_signed(d[q], kP[P]);
+	col_fwd_lo<RL, 0>(u[0], twf[0], tt);
+	col_fwd_lo<RL, 1>(u[1], twf[1], tt);
+	if constexpr (NP == 3) col_fwd_lo<RL, 2>(u[NP - 1], twf[NP - 1], tt);
+	if constexpr (D_REUSED) __syncthreads();
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+	{
+		uint32_t *bw = bufs[P] + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) bw[loff<RL, 0>(q)] = u[P][q];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+	{
+		const uint32_t *br = bufs[P] + cw * G::RP + lbase<RL, 1>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[P][q] = br[loff<RL, 1>(q)];
+	}
+	col_fwd_hi<RL, 0>(u[0], twf[0], tt);
+	col_fwd_hi<RL, 1>(u[1], twf[1], tt);
+	if constexpr (NP == 3) col_fwd_hi<RL, 2>(u[NP - 1], twf[NP - 1], tt);
+#pragma unroll
+	for (int P = 0; P < NP; ++P)
+	{
+		uint32_t *out = res_lane + P * plane + c0 + cw;
+#pragma unroll
+		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, 1>(tt, q) * C] = u[P][q];
+	}
+}
+
 // A: chunk load (+ pending carry), column levels, residue planes.  MINUS: the
 // transform has length 2n (n = nd digits in rows < R/2, zeros below), for the
 // exact product modulo N- = b^nd - b^(nd/2) - 1 (folded in pass C).
@@ -145,6 +202,7 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_arg
 	constexpr uint32_t nd = MINUS ? n / 2 : n, nc = nd / 16, RD = MINUS ? G::R / 2 : G::R;   // digits, chunks, digit rows
 	__shared__ int32_t D[G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
+	__shared__ uint32_t Y[(NP == 3 && G::passes == 2 && !SPLIT) ? G::W * G::RP : 1];    // third transpose plane
 	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
 	if (A.lane_i && A.lane_i[lane] < 0) return;
 	const int32_t *x = (operand ? A.xb : A.x) + (size_t)lane * nd;
@@ -179,7 +237,15 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_arg
 	}
 	__syncthreads();
 	const uint32_t cw = t % G::W, tt = t / G::W;
-	colA_prime_all<LN, RL, NP, W>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n, SPLIT ? (int)blockIdx.z : -1);
+	if constexpr (G::passes == 2 && !SPLIT)
+	{
+		uint32_t *bufs[NP];
+		bufs[0] = X;
+		if constexpr (NP == 3) bufs[1] = Y;
+		bufs[NP - 1] = reinterpret_cast<uint32_t *>(D);
+		colA_all_shared<LN, RL, NP, W, true>(D, bufs, res, A.tw, c0, cw, tt, (size_t)A.lanes * n);
+	}
+	else colA_prime_all<LN, RL, NP, W>(D, X, res, A.tw, c0, cw, tt, (size_t)A.lanes * n, SPLIT ? (int)blockIdx.z : -1);
 }
 
 // B: row levels with the pointwise square / product, in place
@@ -519,57 +585,15 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 #pragma unroll
 	for (int q = 0; q < E; ++q) D[(16 * h + q) * G::RP + r + (r >> 5)] = dig[q];
 	__syncthreads();
-	// forward column levels of every prime around one shared transpose: the
-	// digits are read once, prime P transposes through bufs[P] (X, then the
-	// S[P] that the CRT has finished reading; S[0] holds the digits)
-	const uint32_t cw = t % G::W, tt = t / G::W;
-	int32_t d[E];
-	{
-		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
-#pragma unroll
-		for (int q = 0; q < E; ++q) d[q] = dr[loff<RL, 0>(q)];
-	}
+	// forward column levels of every prime around one shared transpose: prime
+	// P transposes through bufs[P] (X, then the S[P] that the CRT has finished
+	// reading; S[0] holds the digits)
 	uint32_t *bufs[NP];
 	bufs[0] = X;
 #pragma unroll
 	for (int P = 1; P < NP; ++P) bufs[P] = S[P];
-	const tw_entry *twf[NP];
-#pragma unroll
-	for (int P = 0; P < NP; ++P) twf[P] = A.tw + (size_t)(P * 2) * n;
-	uint32_t u[NP][E];
-#pragma unroll
-	for (int P = 0; P < NP; ++P)
-#pragma unroll
-		for (int q = 0; q < E; ++q) u[P][q] = from_signed(d[q], kP[P]);
-	col_fwd_lo<RL, 0>(u[0], twf[0], tt);
-	col_fwd_lo<RL, 1>(u[1], twf[1], tt);
-	if constexpr (NP == 3) col_fwd_lo<RL, 2>(u[NP - 1], twf[NP - 1], tt);
-#pragma unroll
-	for (int P = 0; P < NP; ++P)
-	{
-		uint32_t *bw = bufs[P] + cw * G::RP + lbase<RL, 0>(tt);
-#pragma unroll
-		for (int q = 0; q < E; ++q) bw[loff<RL, 0>(q)] = u[P][q];
-	}
-	__syncthreads();
-#pragma unroll
-	for (int P = 0; P < NP; ++P)
-	{
-		const uint32_t *br = bufs[P] + cw * G::RP + lbase<RL, 1>(tt);
-#pragma unroll
-		for (int q = 0; q < E; ++q) u[P][q] = br[loff<RL, 1>(q)];
-	}
-	col_fwd_hi<RL, 0>(u[0], twf[0], tt);
-	col_fwd_hi<RL, 1>(u[1], twf[1], tt);
-	if constexpr (NP == 3) col_fwd_hi<RL, 2>(u[NP - 1], twf[NP - 1], tt);
-	uint32_t *res_lane = A.res + (size_t)lane * n;
-#pragma unroll
-	for (int P = 0; P < NP; ++P)
-	{
-		uint32_t *out = res_lane + P * (size_t)A.lanes * n + c0 + cw;
-#pragma unroll
-		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, 1>(tt, q) * C] = u[P][q];
-	}
+	colA_all_shared<LN, RL, NP, W, false>(D, bufs, A.res + (size_t)lane * n, A.tw, c0, t % G::W, t / G::W,
+	                                      (size_t)A.lanes * n);
 }
 
 // normalize2 alone: materialises the pending carries into the digits
