@@ -96,8 +96,27 @@ __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool du
 	transform<LN, 0, MUL, MINUS>(D, yd, r0, lds, tw, t);
 	transform<LN, 1, MUL, MINUS>(D, yd, r1, lds, tw, t);
 	if (NP == 3) transform<LN, 2, MUL, MINUS>(D, yd, r2, lds, tw, t);
-	xchg<LN, 0, a>(r0, lds, t);
-	xchg<LN, 0, a>(r1, lds, t);
+	// to the carry layout: primes 0 and 1 share one transpose, prime 1 through
+	// D (every read of the digits came before the first barrier of prime 1's
+	// transform; each thread later writes its carried digits to exactly the
+	// words of D it reads here)
+	{
+		uint32_t *Du = reinterpret_cast<uint32_t *>(D);
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < E; ++q)
+		{
+			lds[lbase<LN, 0>(t) + loff<LN, 0>(q)] = r0[q];
+			Du[lbase<LN, 0>(t) + loff<LN, 0>(q)] = r1[q];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < E; ++q)
+		{
+			r0[q] = lds[lbase<LN, a>(t) + loff<LN, a>(q)];
+			r1[q] = Du[lbase<LN, a>(t) + loff<LN, a>(q)];
+		}
+	}
 	if (NP == 3) xchg<LN, 0, a>(r2, lds, t);
 
 	// normalize1: CRT, optional multiplication by the base, signed digits of 16
