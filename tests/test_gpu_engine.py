@@ -114,6 +114,34 @@ def test_prp_check_config1(kind):
         assert bool(prm[l]) == (want == 1)
 
 
+def test_prp_check_multipass_n2e13():
+    """Full base-2 PRP with Gerbicz-Li at n = 2^13 (~144k-bit N) on the
+    multipass engine -- 256-row columns, graphs, with and without the fused
+    column pass, both column tiles -- against the resident engine on the same
+    candidates: identical canonical residues, RES64 and verdicts.  (GMP's
+    2^(N-1) mod N at this size takes minutes per candidate on one core; the
+    resident engine is pinned to the oracle digit by digit at this n and to
+    GMP on whole PRP tests at n = 2^10.)"""
+    n, ln = 1 << 13, 13
+    rng = random.Random(1313)
+    bs = [rng.randrange(1 << 16, 1 << 17) * 2 for _ in range(2)]
+    print(f"\n[case] prp_check multipass vs resident n=2^13 bs={bs}", flush=True)
+    ref = ct.Transform(ln, len(bs), ct.KIND_RESIDENT)
+    prm0, r640, gl0, res0, rc0 = ct.prp_check(ref, bs, want_residue=True)
+    assert rc0 == 0 and gl0.all()
+    for tile, fused in ((16, 1), (32, 1), (32, 0)):
+        print(f"[case] tile={tile} fused={fused}", flush=True)
+        t = ct.Transform(ln, len(bs), ct.KIND_PLAIN)
+        t.set_column_tile(tile)
+        t.set_fused(fused)
+        prm, r64, gl, res, rc = ct.prp_check(t, bs, want_residue=True)
+        assert rc == 0 and gl.all()
+        assert np.array_equal(res, res0) and np.array_equal(r64, r640) and np.array_equal(prm, prm0)
+    for l, b in enumerate(bs):                         # the residue is canonical: below N, digits in [0, b)
+        assert int(res0[l].max()) < b
+        assert O.digits_value(res0[l], b) < O.modulus(n, b)
+
+
 def test_prp_check_known_prp():
     from sympy import isprime
     n, ln = 32, 5
