@@ -1,7 +1,6 @@
 /* Test infrastructure: GMP's mpz_powm as an independent big-integer oracle for
- * the PRP residues base^(N-1) mod N (the role the SURVEY gives "a GMP mpz_powm
- * oracle at small sizes").  Integers cross the boundary as little-endian
- * byte strings. */
+ * the PRP residues base^(N-1) mod N at small sizes.  Integers cross the
+ * boundary as little-endian byte strings. */
 #include <gmp.h>
 #include <stddef.h>
 #include <stdint.h>
