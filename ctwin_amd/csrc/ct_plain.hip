@@ -534,6 +534,8 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 	const int32_t epoch = (int32_t)((uint32_t)__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
 	                                (uint32_t)A.step + 1u);
 	const uint32_t lane = grp / GW, j = grp % GW, c0 = j * G::W;
+	// an idle lane (per-lane runs): all its groups leave; waits stay within a lane
+	if (A.lane_i && A.lane_i[lane] < 0) return;
 	int32_t dig[E];
 	int64_t f;
 	colC_chunk<LN, RL, NP, W, false>(A, S, X, lane, c0, t, 0, dig, f);      // every row holds digits (N+)
@@ -955,8 +957,8 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 	A.run = e->d_run;
 	hipLaunchKernelGGL(k_run_set, dim3(1), dim3(1), 0, e->st, e->d_run, (int32_t)i_hi);
 	HIPCHK(hipGetLastError());
-	// the fused form: N+, 256-row layout, one exponent-bit index per run
-	const bool fused = e->use_fused && e->sign > 0 && e->rows_log2 == 8 && lane_i == nullptr && nsteps > 1;
+	// the fused form: N+, 256-row layout
+	const bool fused = e->use_fused && e->sign > 0 && e->rows_log2 == 8 && nsteps > 1;
 	if (fused && (e->fused_w != e->col_w || e->fused_np != e->np))
 	{   // a new kernel or grid shape: clear the flags, the words and the epoch
 		// base; tickets only when the grid does not fit the device at once

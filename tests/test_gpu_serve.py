@@ -5,6 +5,8 @@ import random
 
 import pytest
 
+import oracle_lib as O
+
 pytestmark = pytest.mark.gpu
 ct = pytest.importorskip("ctwin_amd")
 
@@ -28,6 +30,14 @@ def python_result(ln, b, steps):
         v, r = divmod(v, b)
         canon.append(r)
     return x % (1 << 64), fnv(canon)
+
+
+def gmp_res64(ln, b, steps):
+    """RES64 (x mod 2^64 of the canonical residue) of the same request by GMP."""
+    n = 1 << ln
+    E = b ** n - b ** (n // 2)
+    N = E + 1
+    return O.powm(2, E >> (E.bit_length() - steps), N) % (1 << 64)
 
 
 def drain(s, want):
@@ -81,6 +91,8 @@ def test_serve_mixed_equals_alone_and_python():
     for rid, ln, b, steps in reqs:
         if ln <= 12:
             assert mixed[rid] == python_result(ln, b, steps), f"request {(rid, ln, b)} vs Python pow"
+        elif ln <= 15:      # multipass classes, per-lane runs (fused column pass where tuned)
+            assert mixed[rid][0] == gmp_res64(ln, b, steps), f"request {(rid, ln, b)} vs GMP"
 
 
 def test_serve_budget_forces_class_recycling():
