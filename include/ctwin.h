@@ -81,9 +81,10 @@ int ct_engine_set_column_split(ct_engine *e, int on);
  * chunk carries through device memory within one launch, or keep three
  * launches per squaring (on = 0).  on = 2 is on = 1 with the workgroups
  * always ordered by ticket (otherwise only when the grid does not fit the
- * device at once).  Results do not depend on it.  A failed exchange is reported by the next
- * read of register 0 and switches the fused form off.  The autotuner (csize
- * 0) chooses it when it is faster.  No counterpart in the reference. */
+ * device at once).  Results do not depend on it.  A failed exchange is
+ * reported by the next read of register 0 and switches the fused form off.
+ * The autotuner (csize 0) chooses it when it is faster.  No counterpart in the
+ * reference. */
 int ct_engine_set_fused(ct_engine *e, int on);
 
 /* What the autotuner chose and measured: the column tile (0 for resident
