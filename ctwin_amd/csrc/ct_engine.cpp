@@ -331,16 +331,19 @@ int ct_engine_moduli(uint32_t *p)
 
 // A tuned multipass engine whose prime count changed: tune its column tile,
 // pass-A split and fused form for the new count (the engine kind stays).
+// Tuning is an optimisation: if it cannot run (e.g. no device memory for
+// its timing engines), the engine keeps its configuration and the call
+// succeeds; the count is not tried again.
 static int retune_for_primes(ct_engine *e)
 {
 	if (!e->tuned || e->kind != CT_KIND_PLAIN || e->np == e->tuned_np) return CT_OK;
+	e->tuned_np = e->np;
 	tune_val v;
 	const int rc = tune(e->device, e->ln, e->lanes, CT_KIND_PLAIN, e->sign, e->np, &v);
-	if (rc) return rc;
 	HIPCHK(hipSetDevice(e->device));
+	if (rc != CT_OK) return CT_OK;
 	e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->fused_w = 0;
 	for (int i = 1; i < CT_TUNE_TIMINGS; ++i) e->tune_ms[i] = v.ms[i];
-	e->tuned_np = e->np;
 	return CT_OK;
 }
 
