@@ -341,7 +341,11 @@ static int retune_for_primes(ct_engine *e)
 	tune_val v;
 	const int rc = tune(e->device, e->ln, e->lanes, CT_KIND_PLAIN, e->sign, e->np, &v);
 	HIPCHK(hipSetDevice(e->device));
-	if (rc != CT_OK) return CT_OK;
+	if (rc != CT_OK)
+	{
+		(void)hipGetLastError();      // a failed allocation must not surface at the next launch check
+		return CT_OK;
+	}
 	e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->fused_w = 0;
 	for (int i = 1; i < CT_TUNE_TIMINGS; ++i) e->tune_ms[i] = v.ms[i];
 	return CT_OK;
