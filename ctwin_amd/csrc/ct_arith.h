@@ -146,6 +146,7 @@ struct divisor
 {
 	uint32_t d; uint32_t v; uint32_t sh; uint32_t b;   // normalised divisor, its 2-by-1 reciprocal, shift, b
 	uint32_t m_lo, m_hi;                               // m = floor((2^64 - 1) / b), the 64-bit reciprocal
+	uint32_t k_lo, k_hi, c_lo, c_hi;                   // K = c b, c = floor(2^62 / b) + 1 (divrem_i62)
 };
 
 CT_HD divisor make_divisor(uint32_t b)
@@ -156,6 +157,9 @@ CT_HD divisor make_divisor(uint32_t b)
 	r.v = (uint32_t)((~(uint64_t)0) / r.d - ((uint64_t)1 << 32));
 	const uint64_t m = (~(uint64_t)0) / b;
 	r.m_lo = (uint32_t)m; r.m_hi = (uint32_t)(m >> 32);
+	const uint64_t c = ((uint64_t)1 << 62) / b + 1, k = c * b;
+	r.c_lo = (uint32_t)c; r.c_hi = (uint32_t)(c >> 32);
+	r.k_lo = (uint32_t)k; r.k_hi = (uint32_t)(k >> 32);
 	return r;
 }
 
@@ -213,6 +217,22 @@ CT_HD uint64_t udiv64_recip(uint64_t u, const divisor &dv, uint32_t *rem)
 	if (r >= dv.b) { r -= dv.b; q += 1; }
 	*rem = r;
 	return q;
+}
+
+// The same truncating division for |a| < 2^62 - b, through a nonnegative
+// dividend: A = a + K (K = c b in (2^62, 2^62 + b]) lies in [0, 2^63); its
+// quotient is floor(a / b) + c and its remainder a mod b in [0, b).  A
+// negative a with a nonzero remainder then takes the truncated quotient (+1)
+// and remainder (-b).  Fewer instructions than dividing |a| and restoring
+// both signs.
+CT_HD int32_t divrem_i62(int64_t &a, const divisor &dv)
+{
+	const uint64_t K = ((uint64_t)dv.k_hi << 32) | dv.k_lo, c = ((uint64_t)dv.c_hi << 32) | dv.c_lo;
+	uint32_t r;
+	const uint64_t q = udiv64_recip((uint64_t)a + K, dv, &r) - c;
+	const bool fix = (a < 0) && (r != 0);
+	a = (int64_t)q + (fix ? 1 : 0);
+	return fix ? (int32_t)r - (int32_t)dv.b : (int32_t)r;
 }
 
 CT_HD int32_t divrem_i64(int64_t &a, const divisor &dv)

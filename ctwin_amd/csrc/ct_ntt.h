@@ -483,13 +483,18 @@ __device__ __forceinline__ int64_t garner2(uint32_t r1, uint32_t r2, const garne
 __device__ __forceinline__ int64_t carry_chunk64(const int64_t (&l)[E], int32_t (&dig)[E], bool dup, int base_m1,
                                                  const divisor &dv)
 {
+	// the coefficient times m = 1, 2 or 3 (m is uniform over a candidate's
+	// threads): m l = m lo(l) + 2^32 m hi(l) mod 2^64, one mad_u64_u32 and one
+	// mul_lo instead of per-lane selects
+	const uint32_t m = dup ? (uint32_t)(base_m1 + 1) : 1u;
 	int64_t acc = 0;
 #pragma unroll
 	for (int q = 0; q < E; ++q)
 	{
-		acc += l[q];
-		if (dup) { acc += l[q]; if (base_m1 == 2) acc += l[q]; }
-		dig[q] = divrem_i64(acc, dv);
+		const uint64_t lu = (uint64_t)l[q];
+		const uint64_t ml = (uint64_t)(uint32_t)lu * m + ((uint64_t)((uint32_t)(lu >> 32) * m) << 32);
+		acc += (int64_t)ml;
+		dig[q] = divrem_i62(acc, dv);               // |acc| < 2^61 (the two-prime bound times 3)
 	}
 	return acc;
 }

@@ -1,6 +1,7 @@
 // Host check of the device arithmetic in ctwin_amd/csrc/ct_arith.h (the same
 // header, compiled by g++): exact truncating division by b against __int128,
-// the lazy Shoup and Montgomery products against their definitions.
+// the biased division for |a| < 2^62, the lazy Shoup and Montgomery products
+// against their definitions.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -31,6 +32,22 @@ int main(int argc, char **argv)
 		const int32_t r = divrem_i64(x, dv);
 		const __int128 q = (__int128)a / b, rr = (__int128)a % b;      // C truncation
 		if (x != (int64_t)q || r != (int32_t)rr) { if (bad++ < 5) std::printf("div a=%lld b=%u\\n", (long long)a, b); }
+		// the biased form, for |a| < 2^62 - b: random values, the edges of the
+		// range, values near multiples of b
+		int64_t a62;
+		const int64_t edge = ((int64_t)1 << 62) - (int64_t)b - 1;
+		switch (it % 5)
+		{
+		case 0: a62 = a % ((int64_t)1 << 61); break;
+		case 1: a62 = edge - (int64_t)(rng() % 1000); break;
+		case 2: a62 = -(edge - (int64_t)(rng() % 1000)); break;
+		case 3: a62 = (int64_t)(rng() % 1000 + 1) * (int64_t)b * ((rng() & 1) ? 1 : -1) + (int64_t)(rng() % 3) - 1; break;
+		default: a62 = a % ((int64_t)1 << 40); break;
+		}
+		int64_t y = a62;
+		const int32_t r62 = divrem_i62(y, dv);
+		const __int128 q62 = (__int128)a62 / b, rr62 = (__int128)a62 % b;
+		if (y != (int64_t)q62 || r62 != (int32_t)rr62) { if (bad++ < 5) std::printf("div62 a=%lld b=%u\n", (long long)a62, b); }
 	}
 	for (int P = 0; P < kPrimes; ++P)
 	{
@@ -49,6 +66,6 @@ int main(int argc, char **argv)
 			if (m == 0 || m >= 2 * p || (((uint64_t)m << 32) % p) != ((uint64_t)x * y) % p) { if (bad++ < 5) std::printf("mont\\n"); }
 		}
 	}
-	std::printf("%ld mismatches\\n", bad);
+	std::printf("%ld mismatches\n", bad);
 	return bad ? 1 : 0;
 }
