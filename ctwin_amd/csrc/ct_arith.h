@@ -100,6 +100,18 @@ CT_HD uint32_t mont_lazy(uint32_t a, uint32_t b, uint32_t p, uint32_t q)
 static_assert(3ull * kP[2] > 0x80000000ull && 3ull * kP[0] < 0x100000000ull, "digit range");
 CT_HD uint32_t from_signed(int32_t d, uint32_t p) { return (d < 0) ? (uint32_t)d + 3 * p : (uint32_t)d; }
 
+// Two-prime mode only: there every digit is below 2^28 in magnitude
+// (3 n D^2 < p1 p2 / 2 with n >= 32 gives D < 2^26.01, and a pending carry
+// adds less than another D), far inside (-2p, 2p), so d + 2p is a
+// representative in [0, 4p) -- the input range of the first forward level --
+// in one instruction.
+template <int NP>
+CT_HD uint32_t from_signed_np(int32_t d, uint32_t p)
+{
+	if constexpr (NP == 2) return (uint32_t)d + 2 * p;
+	else return from_signed(d, p);
+}
+
 // ---- host-side (or constexpr) plain modular helpers -------------------------
 
 CT_HD uint32_t mulmod_plain(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)(((uint64_t)a * b) % p); }

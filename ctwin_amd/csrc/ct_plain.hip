@@ -118,7 +118,7 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
 		const uint32_t p = kP[P];
 #pragma unroll
-		for (int q = 0; q < E; ++q) u[q] = from_signed(dr[loff<RL, 0>(q)], p);
+		for (int q = 0; q < E; ++q) u[q] = from_signed_np<NP>(dr[loff<RL, 0>(q)], p);
 		const tw_entry *twf = tw + (size_t)(P * 2) * n;
 		uint32_t *Xc = X + cw * G::RP;
 		ptw w0;
@@ -157,7 +157,7 @@ __device__ __forceinline__ void colA_all_shared(const int32_t *D, uint32_t *cons
 #pragma unroll
 	for (int P = 0; P < NP; ++P)
 #pragma unroll
-		for (int q = 0; q < E; ++q) u[P][q] = from_signed(d[q], kP[P]);
+		for (int q = 0; q < E; ++q) u[P][q] = from_signed_np<NP>(d[q], kP[P]);
 	col_fwd_lo<RL, 0>(u[0], twf[0], tt);
 	col_fwd_lo<RL, 1>(u[1], twf[1], tt);
 	if constexpr (NP == 3) col_fwd_lo<RL, 2>(u[NP - 1], twf[NP - 1], tt);

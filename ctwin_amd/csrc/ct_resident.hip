@@ -41,7 +41,7 @@ namespace ct {
 // product mod X^(n/2) - X^(n/4) - 1 in slots q < 8; the signed digits are read
 // from LDS (layout 0, region D), the output residues (exact coefficients mod
 // p, in [0, p)) are left in registers in layout 0.
-template <int LN, int P, bool MUL, bool MINUS>
+template <int LN, int P, int NP, bool MUL, bool MINUS>
 __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[E], uint32_t (&out)[E],
                                           uint32_t *lds, const tw_entry *tw, uint32_t t)
 {
@@ -50,13 +50,13 @@ __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[
 	uint32_t u[E], v[E];
 	const int32_t *dr = D + lbase<LN, 0>(t);
 #pragma unroll
-	for (int k = 0; k < E; ++k) u[k] = from_signed(dr[loff<LN, 0>(k)], p);
+	for (int k = 0; k < E; ++k) u[k] = from_signed_np<NP>(dr[loff<LN, 0>(k)], p);
 	ftw<LN> F;
 	fwd_all<LN, P, true, false>(u, lds, twf, twi, 1u, t, F);
 	if (MUL)
 	{
 #pragma unroll
-		for (int k = 0; k < E; ++k) v[k] = from_signed(yd[k], p);
+		for (int k = 0; k < E; ++k) v[k] = from_signed_np<NP>(yd[k], p);
 		fwd_all<LN, P, true, false>(v, lds, twf, twi, 1u, t, F);
 	}
 	else
@@ -93,9 +93,9 @@ __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool du
 	constexpr uint32_t T = geo<LN>::T;
 	constexpr uint32_t TC = MINUS ? T / 2 : T;          // threads that own a chunk of digits
 	uint32_t r0[E], r1[E], r2[E];
-	transform<LN, 0, MUL, MINUS>(D, yd, r0, lds, tw, t);
-	transform<LN, 1, MUL, MINUS>(D, yd, r1, lds, tw, t);
-	if (NP == 3) transform<LN, 2, MUL, MINUS>(D, yd, r2, lds, tw, t);
+	transform<LN, 0, NP, MUL, MINUS>(D, yd, r0, lds, tw, t);
+	transform<LN, 1, NP, MUL, MINUS>(D, yd, r1, lds, tw, t);
+	if (NP == 3) transform<LN, 2, NP, MUL, MINUS>(D, yd, r2, lds, tw, t);
 	// to the carry layout: primes 0 and 1 share one transpose, prime 1 through
 	// D (every read of the digits came before the first barrier of prime 1's
 	// transform; each thread later writes its carried digits to exactly the

@@ -62,3 +62,25 @@ def test_moduli_are_ntt_primes_below_2_30():
             x = pow(a, d, p)
             assert x in (1, p - 1) or any(pow(x, 2 ** k, p) == p - 1 for k in range(1, s))
     assert 3 * (1 << 22) * (1 << 31) ** 2 < P[0] * P[1] * P[2] // 2
+
+
+def test_two_prime_mode_keeps_digits_far_inside_2p():
+    """Two-prime mode converts a signed digit d to d + 2p in one instruction,
+    which needs |d| < 2p and d + 2p < 4p.  The mode is chosen only when
+    3 n D^2 (N+; 4 n D^2 for N-) < p1 p2 / 2, D >= every |digit| and >= b;
+    a pending carry leaves digits below 2 D.  For every supported n >= 32 the
+    largest admissible D keeps 2 D below 2^28, far inside 2 p_min."""
+    if not os.path.exists(ctwin_amd.LIB_PATH):
+        pytest.skip("libctwin_amd.so not built")
+    P = ctwin_amd.moduli()
+    h2 = P[0] * P[1] // 2
+    for ln in range(5, 23):
+        n = 1 << ln
+        d = 1
+        while 3 * n * (2 * d) ** 2 < h2:      # the largest D with 3 n D^2 < p1 p2 / 2: doubling, then bisection
+            d *= 2
+        lo, hi = d, 2 * d                      # 3 n lo^2 < p1 p2 / 2 <= 3 n hi^2
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            lo, hi = (mid, hi) if 3 * n * mid * mid < h2 else (lo, mid)
+        assert 2 * lo < (1 << 28) < 2 * min(P)
