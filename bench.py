@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="multipass engines launch every kernel instead of replaying captured HIP graphs "
                          "(same results; used for kernel traces)")
+    ap.add_argument("--sieve-only", action="store_true",
+                    help="run only the config-2 window sieve and print its JSON object (for profiling the sieve)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for hardware counters: 50 batched steps in one launch, 20 plain "
                          "squarings at n=2^14 and n=2^21, and 3 copies of an 8 MiB register "
@@ -203,6 +205,10 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import ctwin_amd as ct
+
+    if args.sieve_only:
+        print(json.dumps({"sieve_config2": run_sieve_config2(ct)}), flush=True)
+        return
 
     if args.profile:
         run_batched(ct, 50, 1)
