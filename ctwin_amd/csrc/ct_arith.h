@@ -112,6 +112,62 @@ CT_HD uint32_t from_signed_np(int32_t d, uint32_t p)
 	else return from_signed(d, p);
 }
 
+// ---- 32-bit primality -----------------------------------------------------------
+
+// Strong probable-prime test of odd p > 41 to the bases 2, 7 and 61
+// (deterministic below 4,759,123,141, so exact for every 32-bit p; G. Jaeschke,
+// Math. Comp. 61 (1993) 915-926), in 32-bit Montgomery form.  No 64-bit
+// remainder: R = 2^32 mod p is 2^32 - p when p > 2^31 and one 32-bit remainder
+// otherwise, and the bases' Montgomery forms are sums of modular doublings of
+// R (7 = 8 - 1, 61 = 64 - 2 - 1).  spsp_base tests one base (0: 2, 1: 7,
+// 2: 61), so that a caller can test base 2 on every candidate and the others
+// on the few that pass it.
+CT_HD bool spsp_base(uint32_t p, int which)
+{
+	uint32_t q = p;
+	for (int i = 0; i < 5; ++i) q *= 2u - p * q;                    // p^-1 mod 2^32
+	auto mul = [&](uint32_t a, uint32_t b) {
+		const uint64_t t = (uint64_t)a * b;
+		const uint32_t m = (uint32_t)t * q, th = (uint32_t)(t >> 32), mp = mulhi32(m, p);
+		return (th < mp) ? th - mp + p : th - mp;
+	};
+	auto add = [&](uint32_t a, uint32_t b) { const uint32_t t = p - b; return (a >= t) ? a - t : a + b; };
+	auto sub = [&](uint32_t a, uint32_t b) { return (a >= b) ? a - b : a - b + p; };
+	const uint32_t one = (p & 0x80000000u) ? 0u - p : (0u - p) % p;
+	const uint32_t m1 = p - one;                                     // -1
+	const uint32_t b2 = add(one, one);
+	uint32_t a = b2;
+	if (which > 0)
+	{
+		const uint32_t b8 = add(add(b2, b2), add(b2, b2));
+		if (which == 1) a = sub(b8, one);
+		else
+		{
+			uint32_t x = add(b8, b8); x = add(x, x); x = add(x, x);   // 64
+			a = sub(x, add(b2, one));
+		}
+	}
+	if (a == 0) return true;                                         // p divides the base (p = 61)
+	uint32_t d = p - 1;
+	int s = 0;
+	while ((d & 1) == 0) { d >>= 1; ++s; }
+	uint32_t x = one;
+	for (uint32_t e = d; e; e >>= 1)
+	{
+		if (e & 1) x = mul(x, a);
+		a = mul(a, a);
+	}
+	if (x == one || x == m1) return true;
+	for (int r = 1; r < s; ++r)
+	{
+		x = mul(x, x);
+		if (x == m1) return true;
+	}
+	return false;
+}
+
+CT_HD bool spsp_2_7_61(uint32_t p) { return spsp_base(p, 0) && spsp_base(p, 1) && spsp_base(p, 2); }
+
 // ---- host-side (or constexpr) plain modular helpers -------------------------
 
 CT_HD uint32_t mulmod_plain(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)(((uint64_t)a * b) % p); }

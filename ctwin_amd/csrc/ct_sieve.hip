@@ -60,9 +60,7 @@ __device__ __forceinline__ bool is_prime32(uint32_t p)
 	if (p < 43) return p == 2 || p == 3 || p == 5 || p == 7 || p == 11 || p == 13 || p == 17 || p == 19
 	                || p == 23 || p == 29 || p == 31 || p == 37 || p == 41;
 	if ((p & 1) == 0 || small_factor(p)) return false;
-	const uint32_t bases[3] = {2u, 7u, 61u};
-	m32 A(p);
-	return mr_prime<m32, uint32_t>(A, p, bases, 3);
+	return spsp_2_7_61(p);
 }
 
 // wave-aggregated append of p to list (one atomic per wave)
@@ -138,9 +136,21 @@ __global__ void k_roots_pos(const uint32_t *list, const uint32_t *count, int m, 
 // ---------------------------------------------------------------------------
 // - side
 
-__global__ void k_primes_neg(uint64_t k0, uint64_t kcount, uint32_t *list, uint32_t *count)
+// Candidates p = 10k +- 1 (5 a square mod p), tested with the work packed
+// into full wavefronts: trial division by the primes up to 41 on every
+// candidate, the survivors appended to an LDS list; the base-2 strong test on
+// that list; bases 7 and 61 on the base-2 probable primes; then the append of
+// the primes.  (Tested lane by lane, a wavefront would run the base-2
+// exponentiation with about a third of its lanes and bases 7 and 61 with a
+// twentieth.)  The list order is immaterial: every prime strikes its own b.
+__global__ __launch_bounds__(256) void k_primes_neg(uint64_t k0, uint64_t kcount, uint32_t *list, uint32_t *count)
 {
-	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ uint32_t c1[512], c2[512];
+	__shared__ uint32_t n1, n2;
+	const uint32_t t = threadIdx.x;
+	if (t == 0) { n1 = 0; n2 = 0; }
+	__syncthreads();
+	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + t;
 	for (int s = 0; s < 2; ++s)
 	{
 		bool keep = false;
@@ -149,9 +159,24 @@ __global__ void k_primes_neg(uint64_t k0, uint64_t kcount, uint32_t *list, uint3
 		{
 			const uint64_t pp = 10 * (k0 + i) + 2 * s - 1;
 			p = (uint32_t)pp;
-			keep = (pp >= 11) && (pp < 0x100000000ull) && is_prime32(p);
+			if (pp >= 11 && pp < 0x100000000ull)
+				keep = (p < 43) ? (p == 11 || p == 19 || p == 29 || p == 31 || p == 41) : !small_factor(p);
 		}
-		append(keep, p, list, count);
+		append(keep, p, c1, &n1);
+	}
+	__syncthreads();
+	const uint32_t m1 = n1;
+	for (uint32_t j = t; j < m1; j += blockDim.x)
+	{
+		const uint32_t p = c1[j];
+		append(p < 43 || spsp_base(p, 0), p, c2, &n2);
+	}
+	__syncthreads();
+	const uint32_t m2 = n2;
+	for (uint32_t j = t; j < m2; j += blockDim.x)
+	{
+		const uint32_t p = c2[j];
+		append(p < 43 || (spsp_base(p, 1) && spsp_base(p, 2)), p, list, count);
 	}
 }
 

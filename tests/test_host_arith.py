@@ -18,3 +18,17 @@ def test_device_arithmetic_on_host(tmp_path):
     out = subprocess.run([exe, "1000000"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout
     assert out.stdout.strip().startswith("0 mismatches")
+
+
+def test_32bit_primality_on_host(tmp_path):
+    """The sieve's 32-bit primality test (strong probable primes to 2, 7, 61 in
+    Montgomery form) against exact primality: every odd p < 2^22, random
+    32-bit values, the edges of the ranges, strong pseudoprimes."""
+    exe = str(tmp_path / "primality_check")
+    src = os.path.join(HERE, "host", "primality_check.cpp")
+    r = subprocess.run(["g++", "-O2", "-std=c++17", src, "-o", exe], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.fail(r.stderr)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout
+    assert out.stdout.strip().startswith("0 mismatches")
