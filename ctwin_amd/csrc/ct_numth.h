@@ -85,13 +85,6 @@ CT_HD2 bool mr_prime(const M &A, U p, const uint32_t *bases, int nb)
 	return true;
 }
 
-// a * b mod q for a, b < q (q < 2^63 on the host, < 2^32 on the device)
-CT_HD2 uint64_t mulmod_q(uint64_t a, uint64_t b, uint64_t q)
-{
-	if (q <= 0xffffffffull) return (a * b) % q;
-	return (uint64_t)(((unsigned __int128)a * b) % q);
-}
-
 // Generator of the 2-Sylow subgroup (order 2^v), from the first quadratic
 // non-residue c: w = c^q, p - 1 = 2^v q.
 template <class M, typename U>
@@ -111,10 +104,10 @@ CT_HD2 U root_2d(const M &A, U p, U q, int v, U w, U y, int d)
 {
 	const int e = (v < d) ? v : d;
 	if (A.pow(y, (p - 1) >> e) != A.one) return 0;
-	// alpha = 2^-d mod q
+	// alpha = 2^-d mod q (q odd) by d halvings modulo q: a/2 for even a,
+	// (a + q)/2 = floor(a/2) + floor(q/2) + 1 for odd a -- no remainders
 	uint64_t alpha = 1 % (uint64_t)q;
-	const uint64_t h = ((uint64_t)q + 1) / 2;     // 2^-1 mod q
-	for (int i = 0; i < d; ++i) alpha = mulmod_q(alpha, h, (uint64_t)q);
+	for (int i = 0; i < d; ++i) alpha = (alpha & 1) ? (alpha >> 1) + ((uint64_t)q >> 1) + 1 : alpha >> 1;
 	const U x1 = A.pow(y, alpha);
 	if (d >= v) return x1;          // y lies in the odd part: x1^(2^d) = y
 	// eta = y * x1^(-2^d) lies in the 2-Sylow subgroup and is a 2^d-th power there
