@@ -3,7 +3,12 @@
 // host), Miller-Rabin, the 2-Sylow generator and 2^d-th roots mod p.
 #pragma once
 
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#define CT_HD2 __host__ __device__ __forceinline__
+#else
+#define CT_HD2 inline
+#endif
 
 #include <cstdint>
 
@@ -13,6 +18,7 @@ namespace ct {
 // modular arithmetic: 32-bit Montgomery on the device (odd p < 2^32),
 // plain 128-bit arithmetic on the host (odd p < 2^63); same interface.
 
+#if defined(__HIPCC__)
 struct m32
 {
 	uint32_t p, q, one, r2;
@@ -45,6 +51,8 @@ struct m32
 	}
 };
 
+#endif
+
 struct m64h
 {
 	uint64_t p, one;
@@ -63,8 +71,6 @@ struct m64h
 	}
 };
 
-
-#define CT_HD2 __host__ __device__ __forceinline__
 
 // Miller-Rabin with the given bases.
 template <class M, typename U>
@@ -85,53 +91,110 @@ CT_HD2 bool mr_prime(const M &A, U p, const uint32_t *bases, int nb)
 	return true;
 }
 
-// Generator of the 2-Sylow subgroup (order 2^v), from the first quadratic
-// non-residue c: w = c^q, p - 1 = 2^v q.
+// Quadratic-residue masks modulo the odd primes below 64: bit r is set when r
+// is a nonzero square modulo c.
+constexpr uint64_t qr_mask(uint32_t c)
+{
+	uint64_t m = 0;
+	for (uint32_t x = 1; x < c; ++x) m |= 1ull << (x * x % c);
+	return m;
+}
+constexpr uint32_t kOddPrimes64[17] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61};
+constexpr uint64_t kQrMask64[17] = {qr_mask(3), qr_mask(5), qr_mask(7), qr_mask(11), qr_mask(13), qr_mask(17),
+                                    qr_mask(19), qr_mask(23), qr_mask(29), qr_mask(31), qr_mask(37), qr_mask(41),
+                                    qr_mask(43), qr_mask(47), qr_mask(53), qr_mask(59), qr_mask(61)};
+
+// Generator of the 2-Sylow subgroup (order 2^v) of an odd prime p: w = c^q,
+// p - 1 = 2^v q, c the least quadratic non-residue.  c is prime (a product of
+// residues is a residue).  Below 64 its character comes from p mod c:
+// (2/p) = -1 iff p = 3, 5 mod 8, and for odd c != p quadratic reciprocity gives
+// (c/p) = (p/c) (-1)^((c-1)/2 (p-1)/2).  Past 61 the Euler criterion decides.
 template <class M, typename U>
 CT_HD2 U two_sylow_gen(const M &A, U p, U q)
 {
-	for (uint32_t c = 2;; ++c)
+	const uint32_t p8 = (uint32_t)(p & 7u);
+	if (p8 == 3 || p8 == 5) return A.pow(A.to_m((U)2), q);
+	const bool p3 = (p & 3u) == 3;
+#pragma unroll
+	for (int i = 0; i < 17; ++i)
+	{
+		const uint32_t c = kOddPrimes64[i];
+		const uint32_t r = (uint32_t)(p % c);
+		if (r == 0) continue;                              // c = p
+		const bool res = (kQrMask64[i] >> r) & 1u;          // (p/c) = 1
+		const bool flip = ((c & 3u) == 3) && p3;
+		if (res == flip) return A.pow(A.to_m((U)c), q);
+	}
+	for (uint32_t c = 67;; ++c)
 	{
 		const U cm = A.to_m((U)c);
 		if (A.pow(cm, (p - 1) / 2) == A.neg_one()) return A.pow(cm, q);
 	}
 }
 
+// a^e1 and a^e2 over one squaring chain.
+template <class M, typename U>
+CT_HD2 void pow_pair(const M &A, U a, uint64_t e1, uint64_t e2, U &r1, U &r2)
+{
+	U x = A.one, y = A.one;
+	while (e1 | e2)
+	{
+		if (e1 & 1) x = A.mul(x, a);
+		if (e2 & 1) y = A.mul(y, a);
+		e1 >>= 1; e2 >>= 1;
+		if (e1 | e2) a = A.mul(a, a);
+	}
+	r1 = x; r2 = y;
+}
+
 // One solution of x^(2^d) = y (Montgomery form), or 0 when there is none.
 // p - 1 = 2^v q, q odd, w generator of the 2-Sylow subgroup.
+//
+// With e = min(v, d), y is a 2^d-th power iff y^((p-1)/2^e) = z^(2^(v-e)) = 1,
+// z = y^q.  x1 = y^alpha, alpha = 2^-d mod q, is a 2^d-th root of y up to the
+// 2-Sylow factor eta = y x1^(-2^d) = y^(1 - alpha 2^d) = z^(-c), where
+// alpha 2^d = 1 + c q; c is collected by the halvings that make alpha.  eta =
+// W^k with W = w^(2^d) of order 2^L, L = v - d, and the root is x1 w^k.
 template <class M, typename U>
 CT_HD2 U root_2d(const M &A, U p, U q, int v, U w, U y, int d)
 {
 	const int e = (v < d) ? v : d;
-	if (A.pow(y, (p - 1) >> e) != A.one) return 0;
-	// alpha = 2^-d mod q (q odd) by d halvings modulo q: a/2 for even a,
-	// (a + q)/2 = floor(a/2) + floor(q/2) + 1 for odd a -- no remainders
+	// alpha by d halvings modulo q: a/2 for even a, (a + q)/2 = floor(a/2) +
+	// floor(q/2) + 1 for odd a, which adds 2^i to c -- no remainders
 	uint64_t alpha = 1 % (uint64_t)q;
-	for (int i = 0; i < d; ++i) alpha = (alpha & 1) ? (alpha >> 1) + ((uint64_t)q >> 1) + 1 : alpha >> 1;
-	const U x1 = A.pow(y, alpha);
+	int64_t c = (q == 1) ? -1 : 0;
+	for (int i = 0; i < d; ++i)
+	{
+		if (alpha & 1) { alpha = (alpha >> 1) + ((uint64_t)q >> 1) + 1; c += (int64_t)1 << i; }
+		else alpha >>= 1;
+	}
+	U z, x1;
+	pow_pair(A, y, (uint64_t)q, alpha, z, x1);
+	U t = z;
+	for (int i = 0; i < v - e; ++i) t = A.mul(t, t);
+	if (t != A.one) return 0;
 	if (d >= v) return x1;          // y lies in the odd part: x1^(2^d) = y
-	// eta = y * x1^(-2^d) lies in the 2-Sylow subgroup and is a 2^d-th power there
-	U x1d = x1;
-	for (int i = 0; i < d; ++i) x1d = A.mul(x1d, x1d);
-	const U eta = A.mul(y, A.pow(x1d, p - 2));
-	// eta = W^k, W = w^(2^d) of order 2^L, L = v - d: bits of k from the bottom
+	const uint64_t vmask = ((uint64_t)1 << v) - 1;
+	const U eta = A.pow(z, (uint64_t)(-c) & vmask);
 	const int L = v - d;
 	U W = w;
 	for (int i = 0; i < d; ++i) W = A.mul(W, W);
-	U Wi = A.pow(W, p - 2);         // W^-1, squared along
+	U Wi = W;                       // W^-1 = W^(2^L - 1), by x <- x^2 W; squared along
+	for (int i = 1; i < L; ++i) Wi = A.mul(A.mul(Wi, Wi), W);
 	U mu = eta;
 	uint64_t k = 0;
 	for (int j = 0; j < L; ++j)
 	{
-		U t = mu;
-		for (int i = 0; i < L - 1 - j; ++i) t = A.mul(t, t);
-		if (t != A.one) { k |= (uint64_t)1 << j; mu = A.mul(mu, Wi); }
+		U s = mu;
+		for (int i = 0; i < L - 1 - j; ++i) s = A.mul(s, s);
+		if (s != A.one) { k |= (uint64_t)1 << j; mu = A.mul(mu, Wi); }
 		Wi = A.mul(Wi, Wi);
 	}
 	return A.mul(x1, A.pow(w, k));
 }
 
 
+#if defined(__HIPCC__)
 // 64-bit Montgomery (R = 2^64) for odd p < 2^63, device side.
 struct m64d
 {
@@ -166,5 +229,7 @@ struct m64d
 		return r;
 	}
 };
+
+#endif
 
 }   // namespace ct

@@ -1,0 +1,127 @@
+// Host check of the sieve's root construction in ctwin_amd/csrc/ct_numth.h (the
+// same header, compiled by g++ with its 128-bit host arithmetic):
+//  - two_sylow_gen picks c^q for the least quadratic non-residue c, found by a
+//    scan of every integer c = 2, 3, ... with the Euler criterion;
+//  - root_2d returns the same root as the direct construction below (Euler
+//    check by y^((p-1)/2^e), inversions by Fermat's little theorem), and
+//    the root it returns satisfies x^(2^d) = y; 0 exactly when y is no
+//    2^d-th power.
+// Over every prime p = 10k +- 1 below 2^18, the primes p = 1 mod 2^12 below
+// 2^26, the Fermat primes, and random primes below 2^32 and 2^62.
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "../../ctwin_amd/csrc/ct_numth.h"
+
+using namespace ct;
+
+static uint64_t gen_scan(const m64h &A, uint64_t p, uint64_t q)
+{
+	for (uint64_t c = 2;; ++c)
+		if (A.pow(c % p, (p - 1) / 2) == p - 1) return A.pow(c % p, q);
+}
+
+static uint64_t root_direct(const m64h &A, uint64_t p, uint64_t q, int v, uint64_t w, uint64_t y, int d)
+{
+	const int e = (v < d) ? v : d;
+	if (A.pow(y, (p - 1) >> e) != 1) return 0;
+	uint64_t alpha = 1 % q;
+	for (int i = 0; i < d; ++i) alpha = (alpha & 1) ? (alpha + q) / 2 : alpha / 2;
+	const uint64_t x1 = A.pow(y, alpha);
+	if (d >= v) return x1;
+	uint64_t x1d = x1;
+	for (int i = 0; i < d; ++i) x1d = A.mul(x1d, x1d);
+	const uint64_t eta = A.mul(y, A.pow(x1d, p - 2));
+	const int L = v - d;
+	uint64_t W = w;
+	for (int i = 0; i < d; ++i) W = A.mul(W, W);
+	uint64_t Wi = A.pow(W, p - 2), mu = eta, k = 0;
+	for (int j = 0; j < L; ++j)
+	{
+		uint64_t t = mu;
+		for (int i = 0; i < L - 1 - j; ++i) t = A.mul(t, t);
+		if (t != 1) { k |= 1ull << j; mu = A.mul(mu, Wi); }
+		Wi = A.mul(Wi, Wi);
+	}
+	return A.mul(x1, A.pow(w, k));
+}
+
+static bool is_prime(uint64_t n)
+{
+	if (n < 2) return false;
+	static const uint32_t b[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+	for (uint32_t s : b) { if (n == s) return true; if (n % s == 0) return false; }
+	return mr_prime<m64h, uint64_t>(m64h(n), n, b, 12);
+}
+
+static long bad = 0, checked = 0, roots = 0;
+
+static void check_prime(uint64_t p, std::mt19937_64 &rng, int ny)
+{
+	const m64h A(p);
+	uint64_t q = p - 1; int v = 0;
+	while ((q & 1) == 0) { q >>= 1; ++v; }
+	const uint64_t w = two_sylow_gen<m64h, uint64_t>(A, p, q);
+	if (w != gen_scan(A, p, q)) { if (bad++ < 10) std::printf("gen p=%llu\n", (unsigned long long)p); }
+	static const int ds[] = {1, 2, 5, 11, 12, 20};
+	for (int d : ds)
+	{
+		for (int t = 0; t < ny; ++t)
+		{
+			uint64_t y;
+			if (t == 0) y = 5 % p;
+			else if (t == 1) y = 1;
+			else if (t == 2) y = p - 1;
+			else if (t == 3) { y = rng() % p; for (int i = 0; i < d; ++i) y = A.mul(y, y); }   // a 2^d-th power
+			else y = rng() % p;
+			if (y == 0) continue;
+			++checked;
+			const uint64_t x = root_2d<m64h, uint64_t>(A, p, q, v, w, y, d);
+			const uint64_t want = root_direct(A, p, q, v, w, y, d);
+			bool ok = (x == want);
+			if (x != 0)
+			{
+				uint64_t s = x;
+				for (int i = 0; i < d; ++i) s = A.mul(s, s);
+				ok = ok && (s == y);
+				++roots;
+			}
+			else
+			{
+				const int e = (v < d) ? v : d;
+				ok = ok && (A.pow(y, (p - 1) >> e) != 1);
+			}
+			if (!ok && bad++ < 10)
+				std::printf("root p=%llu d=%d y=%llu got=%llu want=%llu\n", (unsigned long long)p, d,
+				            (unsigned long long)y, (unsigned long long)x, (unsigned long long)want);
+		}
+	}
+}
+
+int main()
+{
+	std::mt19937_64 rng(20241);
+	const uint32_t L = 1u << 18;
+	std::vector<uint8_t> comp(L, 0);
+	for (uint32_t i = 2; (uint64_t)i * i < L; ++i)
+		if (!comp[i])
+			for (uint32_t j = i * i; j < L; j += i) comp[j] = 1;
+	for (uint32_t p = 11; p < L; p += 2)
+		if (!comp[p] && (p % 10 == 1 || p % 10 == 9)) check_prime(p, rng, 5);
+	for (uint64_t p = (1u << 12) + 1; p < (1u << 26); p += 1u << 12)
+		if (is_prime(p)) check_prime(p, rng, 6);
+	for (uint64_t p : {3ull, 5ull, 17ull, 257ull, 65537ull}) check_prime(p, rng, 6);
+	for (int i = 0; i < 3000;)
+	{
+		const uint64_t p = (rng() >> 32) | 1;
+		if (p > 61 && is_prime(p)) { check_prime(p, rng, 6); ++i; }
+	}
+	for (int i = 0; i < 300;)
+	{
+		const uint64_t p = (rng() >> 2) | 1;
+		if (is_prime(p)) { check_prime(p, rng, 6); ++i; }
+	}
+	std::printf("%ld mismatches in %ld roots (%ld found)\n", bad, checked, roots);
+	return bad ? 1 : 0;
+}
