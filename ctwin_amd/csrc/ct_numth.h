@@ -147,40 +147,55 @@ CT_HD2 void pow_pair(const M &A, U a, uint64_t e1, uint64_t e2, U &r1, U &r2)
 	r1 = x; r2 = y;
 }
 
-// One solution of x^(2^d) = y (Montgomery form), or 0 when there is none.
-// p - 1 = 2^v q, q odd, w generator of the 2-Sylow subgroup.
-//
-// With e = min(v, d), y is a 2^d-th power iff y^((p-1)/2^e) = z^(2^(v-e)) = 1,
-// z = y^q.  x1 = y^alpha, alpha = 2^-d mod q, is a 2^d-th root of y up to the
-// 2-Sylow factor eta = y x1^(-2^d) = y^(1 - alpha 2^d) = z^(-c), where
-// alpha 2^d = 1 + c q; c is collected by the halvings that make alpha.  eta =
-// W^k with W = w^(2^d) of order 2^L, L = v - d, and the root is x1 w^k.
-template <class M, typename U>
-CT_HD2 U root_2d(const M &A, U p, U q, int v, U w, U y, int d)
+// alpha = 2^-d mod q (q odd) by d halvings modulo q: a/2 for even a,
+// (a + q)/2 = floor(a/2) + floor(q/2) + 1 for odd a, which adds 2^i to c, so
+// that alpha 2^d = 1 + c q -- no remainders.
+CT_HD2 void inv_pow2_mod(uint64_t q, int d, uint64_t &alpha, int64_t &c)
 {
-	const int e = (v < d) ? v : d;
-	// alpha by d halvings modulo q: a/2 for even a, (a + q)/2 = floor(a/2) +
-	// floor(q/2) + 1 for odd a, which adds 2^i to c -- no remainders
-	uint64_t alpha = 1 % (uint64_t)q;
-	int64_t c = (q == 1) ? -1 : 0;
+	alpha = (q == 1) ? 0 : 1;                 // 1 mod q
+	c = (q == 1) ? -1 : 0;
 	for (int i = 0; i < d; ++i)
 	{
-		if (alpha & 1) { alpha = (alpha >> 1) + ((uint64_t)q >> 1) + 1; c += (int64_t)1 << i; }
+		if (alpha & 1) { alpha = (alpha >> 1) + (q >> 1) + 1; c += (int64_t)1 << i; }
 		else alpha >>= 1;
 	}
-	U z, x1;
-	pow_pair(A, y, (uint64_t)q, alpha, z, x1);
+}
+
+// z^-1 for z in the 2-Sylow subgroup (z^(2^v) = 1): z^(2^v - 1) by x <- x^2 z.
+template <class M, typename U>
+CT_HD2 U inv_2sylow(const M &A, U z, int v)
+{
+	U x = z;
+	for (int i = 1; i < v; ++i) x = A.mul(A.mul(x, x), z);
+	return x;
+}
+
+// Whether y is a 2^d-th power, from z = y^q: y^((p-1)/2^e) = z^(2^(v-e)) = 1,
+// e = min(v, d).
+template <class M, typename U>
+CT_HD2 bool is_pow2d(const M &A, U z, int v, int d)
+{
+	const int e = (v < d) ? v : d;
 	U t = z;
 	for (int i = 0; i < v - e; ++i) t = A.mul(t, t);
-	if (t != A.one) return 0;
+	return t == A.one;
+}
+
+// A 2^d-th root of y (a 2^d-th power, Montgomery form) from x1 = y^alpha and
+// z = y^q (alpha 2^d = 1 + c q).  p - 1 = 2^v q, q odd, w generator of the
+// 2-Sylow subgroup.  x1 is a 2^d-th root of y up to the 2-Sylow factor
+// eta = y x1^(-2^d) = y^(1 - alpha 2^d) = z^(-c); eta = W^k with W = w^(2^d)
+// of order 2^L, L = v - d, and the root is x1 w^k.
+template <class M, typename U>
+CT_HD2 U root_2d_finish(const M &A, int v, U w, U x1, U z, int64_t c, int d)
+{
 	if (d >= v) return x1;          // y lies in the odd part: x1^(2^d) = y
 	const uint64_t vmask = ((uint64_t)1 << v) - 1;
 	const U eta = A.pow(z, (uint64_t)(-c) & vmask);
 	const int L = v - d;
 	U W = w;
 	for (int i = 0; i < d; ++i) W = A.mul(W, W);
-	U Wi = W;                       // W^-1 = W^(2^L - 1), by x <- x^2 W; squared along
-	for (int i = 1; i < L; ++i) Wi = A.mul(A.mul(Wi, Wi), W);
+	U Wi = inv_2sylow(A, W, L);     // W^-1, squared along
 	U mu = eta;
 	uint64_t k = 0;
 	for (int j = 0; j < L; ++j)
@@ -191,6 +206,28 @@ CT_HD2 U root_2d(const M &A, U p, U q, int v, U w, U y, int d)
 		Wi = A.mul(Wi, Wi);
 	}
 	return A.mul(x1, A.pow(w, k));
+}
+
+// One solution of x^(2^d) = y (Montgomery form), or 0 when there is none.
+// z = y^q and x1 = y^alpha share one squaring chain; for d = 1, alpha =
+// (q + 1)/2 and both come from t = y^((q-1)/2): x1 = t y, z = t x1.
+template <class M, typename U>
+CT_HD2 U root_2d(const M &A, U p, U q, int v, U w, U y, int d)
+{
+	(void)p;
+	uint64_t alpha;
+	int64_t c;
+	inv_pow2_mod((uint64_t)q, d, alpha, c);
+	U z, x1;
+	if (d == 1 && q > 1)
+	{
+		const U t = A.pow(y, ((uint64_t)q - 1) / 2);
+		x1 = A.mul(t, y);
+		z = A.mul(t, x1);
+	}
+	else pow_pair(A, y, (uint64_t)q, alpha, z, x1);
+	if (!is_pow2d(A, z, v, d)) return 0;
+	return root_2d_finish(A, v, w, x1, z, c, d);
 }
 
 

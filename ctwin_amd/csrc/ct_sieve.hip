@@ -8,16 +8,18 @@
 // The reference filters p with a base-2 Euler test and verifies every strike;
 // a composite p that survives it can only strike b that share a smaller prime
 // factor with it, so the survivor set is the same as with exact primes.  Here
-// primality is exact (Miller-Rabin to bases 2, 7, 61 is deterministic below
-// 4,759,123,141) and the roots are computed directly:
+// primality is exact (+: Miller-Rabin to bases 2, 7, 61, deterministic below
+// 4,759,123,141; -: a sieve of Eratosthenes) and the roots are computed
+// directly:
 //   + : c = a^k, k = (p-1)/(3 2^m), for the first a with c^(3 2^(m-1)) = -1
 //       and c^(2^(m-1)) != -1 (then c has order 3 2^m); roots c g^t and c^5 g^t,
 //       g = c^6, t < 2^(m-1).
 //   - : x^(2^d) = y (d = m-1) solved once (y^alpha with alpha 2^d = 1 mod the
 //       odd part of p-1, then a 2-group discrete logarithm), all 2^e roots by
 //       a primitive 2^e-th root of unity (e = min(v2(p-1), d)).
-// Pipeline per block of k: candidate kernel (trial division + Miller-Rabin,
-// wave-aggregated compaction of the primes), root kernels that write 1 into
+// Pipeline per block of k: a candidate kernel (+: trial division and
+// Miller-Rabin with wave-aggregated compaction; -: a segmented sieve of
+// Eratosthenes over k), root kernels that write 1 into
 // struck[b - b_min] (idempotent plain byte stores; no atomics needed).  Primes
 // below 2^16 on the - side strike too many b per root for one thread: their
 // (p, root) pairs are queued and struck by one workgroup each.
@@ -136,48 +138,141 @@ __global__ void k_roots_pos(const uint32_t *list, const uint32_t *count, int m, 
 // ---------------------------------------------------------------------------
 // - side
 
-// Candidates p = 10k +- 1 (5 a square mod p), tested with the work packed
-// into full wavefronts: trial division by the primes up to 41 on every
-// candidate, the survivors appended to an LDS list; the base-2 strong test on
-// that list; bases 7 and 61 on the base-2 probable primes; then the append of
-// the primes.  (Tested lane by lane, a wavefront would run the base-2
-// exponentiation with about a third of its lanes and bases 7 and 61 with a
-// twentieth.)  The list order is immaterial: every prime strikes its own b.
-__global__ __launch_bounds__(256) void k_primes_neg(uint64_t k0, uint64_t kcount, uint32_t *list, uint32_t *count)
+// Primes p = 10k +- 1 (5 a square mod p) by a sieve of Eratosthenes over k,
+// one workgroup per segment of kSegK values of k, with a bitmap per side in
+// LDS (bit k: p = 10k - 1 for side 0, 10k + 1 for side 1).  The sieving
+// primes are every prime 3 <= q < 2^16 but 5 (q = 2 and q = 5 divide no
+// 10k +- 1; every composite p < 2^32 has a prime factor below 2^16).  q
+// divides 10k - 1 exactly for k = r0 mod q, r0 = 10^-1 mod q, and 10k + 1 for
+// k = r1 = q - r0, so each q strikes one residue class of k per side; the
+// strike of p = q itself is skipped.  Sieving primes below kCoopQ stride their
+// many multiples with the whole workgroup; the others take one lane each.  The
+// unstruck k become primes, gathered in an LDS list and appended to the global
+// list with one atomic per workgroup (a list that outgrows LDS, possible only
+// in the first segments where primes are dense, appends its excess directly).
+// The list order is immaterial: every prime strikes its own b.
+struct sieve_prime { uint32_t q, r; };    // r = r0 | r1 << 16
+
+constexpr uint32_t kSegK = 1u << 15;      // k values per workgroup
+constexpr uint32_t kSegCap = 12288;       // LDS prime list (2 kSegK candidates hold about 7,600 primes near 2^32)
+constexpr uint32_t kCoopQ = 256;
+constexpr uint32_t kCoopMax = 64;         // sieving primes below kCoopQ: 52
+
+__device__ __forceinline__ void strike_k(uint32_t *bm, uint32_t k) { atomicOr(&bm[k >> 5], 1u << (k & 31)); }
+
+__global__ __launch_bounds__(256) void k_sieve_neg(uint64_t k0, uint64_t kcount, const sieve_prime *sp, uint32_t nsp,
+                                                  uint32_t ncoop, uint32_t *list, uint32_t *count)
 {
-	__shared__ uint32_t c1[512], c2[512];
-	__shared__ uint32_t n1, n2;
-	const uint32_t t = threadIdx.x;
-	if (t == 0) { n1 = 0; n2 = 0; }
-	__syncthreads();
-	const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + t;
-	for (int s = 0; s < 2; ++s)
+	__shared__ uint32_t bm[2][kSegK / 32];
+	__shared__ uint32_t off[2][kCoopMax];
+	__shared__ uint32_t plist[kSegCap];
+	__shared__ uint32_t npl, gbase;
+	const uint32_t t = threadIdx.x, nt = blockDim.x;
+	const uint64_t kseg = k0 + (uint64_t)blockIdx.x * kSegK;       // below 2^29: p < 2^32
+	const uint64_t kend = min(k0 + kcount, kseg + kSegK);
+	const uint32_t ks = (uint32_t)kseg;
+	for (uint32_t i = t; i < 2 * (kSegK / 32); i += nt) (&bm[0][0])[i] = 0;
+	if (t == 0) npl = 0;
+	for (uint32_t i = t; i < ncoop; i += nt)
 	{
-		bool keep = false;
-		uint32_t p = 0;
-		if (i < kcount)
+		const uint32_t q = sp[i].q, r = sp[i].r, kr = ks % q;
+		for (int s = 0; s < 2; ++s)
 		{
-			const uint64_t pp = 10 * (k0 + i) + 2 * s - 1;
-			p = (uint32_t)pp;
-			if (pp >= 11 && pp < 0x100000000ull)
-				keep = (p < 43) ? (p == 11 || p == 19 || p == 29 || p == 31 || p == 41) : !small_factor(p);
+			uint32_t o = (s ? (r >> 16) : (r & 0xffffu)) + q - kr;
+			off[s][i] = (o >= q) ? o - q : o;
 		}
-		append(keep, p, c1, &n1);
 	}
 	__syncthreads();
-	const uint32_t m1 = n1;
-	for (uint32_t j = t; j < m1; j += blockDim.x)
+	for (uint32_t i = 0; i < ncoop; ++i)
 	{
-		const uint32_t p = c1[j];
-		append(p < 43 || spsp_base(p, 0), p, c2, &n2);
+		const uint32_t q = sp[i].q;
+		for (int s = 0; s < 2; ++s)
+		{
+			const uint32_t o = off[s][i];
+			for (uint32_t k = o + t * q; k < kSegK; k += nt * q)
+			{
+				if (k == o && 10 * (kseg + k) + 2 * s - 1 == q) continue;     // p = q
+				strike_k(bm[s], k);
+			}
+		}
+	}
+	for (uint32_t i = ncoop + t; i < nsp; i += nt)
+	{
+		const uint32_t q = sp[i].q, r = sp[i].r, kr = ks % q;
+		for (int s = 0; s < 2; ++s)
+		{
+			uint32_t o = (s ? (r >> 16) : (r & 0xffffu)) + q - kr;
+			if (o >= q) o -= q;
+			uint32_t k = o;
+			if (k < kSegK && 10 * (kseg + k) + 2 * s - 1 == q) k += q;      // p = q
+			for (; k < kSegK; k += q) strike_k(bm[s], k);
+		}
 	}
 	__syncthreads();
-	const uint32_t m2 = n2;
-	for (uint32_t j = t; j < m2; j += blockDim.x)
+	const int lane = __lane_id();
+	for (uint32_t wi = t; wi < 2 * (kSegK / 32); wi += nt)        // uniform trip count: 2 kSegK / 32 is a multiple of nt
 	{
-		const uint32_t p = c2[j];
-		append(p < 43 || (spsp_base(p, 1) && spsp_base(p, 2)), p, list, count);
+		const int s = (int)(wi / (kSegK / 32));
+		const uint32_t w = wi % (kSegK / 32);
+		uint32_t u = ~bm[s][w];
+		while (__ballot(u != 0))
+		{
+			bool keep = false;
+			uint32_t p = 0;
+			if (u)
+			{
+				const uint32_t b = __ffs(u) - 1;
+				u &= u - 1;
+				const uint64_t k = kseg + w * 32 + b;
+				const uint64_t pp = 10 * k + 2 * s - 1;
+				keep = k < kend && pp >= 11 && pp < 0x100000000ull;
+				p = (uint32_t)pp;
+			}
+			const uint64_t mask = __ballot(keep);
+			if (mask == 0) continue;
+			const int leader = __ffsll((long long)mask) - 1;
+			uint32_t base = 0;
+			if (lane == leader) base = atomicAdd(&npl, (uint32_t)__popcll(mask));
+			base = __shfl(base, leader);
+			if (keep)
+			{
+				const uint32_t slot = base + __popcll(mask & ((1ull << lane) - 1));
+				if (slot < kSegCap) plist[slot] = p;
+				else list[atomicAdd(count, 1u)] = p;
+			}
+		}
 	}
+	__syncthreads();
+	const uint32_t n = min(npl, kSegCap);
+	if (t == 0) gbase = atomicAdd(count, n);
+	__syncthreads();
+	for (uint32_t j = t; j < n; j += nt) list[gbase + j] = plist[j];
+}
+
+// The sieving primes, built once on the host.
+static const std::vector<sieve_prime> &sieve_primes_neg(uint32_t *ncoop)
+{
+	static std::vector<sieve_prime> v;
+	static uint32_t nc = 0;
+	if (v.empty())
+	{
+		const uint32_t L = 1u << 16;
+		std::vector<uint8_t> comp(L, 0);
+		for (uint32_t i = 2; i * i < L; ++i)
+			if (!comp[i])
+				for (uint32_t j = i * i; j < L; j += i) comp[j] = 1;
+		for (uint32_t q = 3; q < L; ++q)
+		{
+			if (comp[q] || q == 5) continue;
+			uint32_t r0 = 0;
+			for (uint32_t m = 0; m < 10; ++m)
+				if ((q * m + 1) % 10 == 0) { r0 = (q * m + 1) / 10; break; }   // 10 r0 = 1 mod q
+			v.push_back(sieve_prime{q, r0 | ((q - r0) << 16)});
+			if (q < kCoopQ) ++nc;
+		}
+	}
+	*ncoop = nc;
+	return v;
 }
 
 constexpr uint32_t kSmallP = 1u << 16;
@@ -199,11 +294,24 @@ __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, 
 	const int d = m - 1, e = (v < d) ? v : d;
 	uint32_t u = w;                                        // primitive 2^e-th root: w^(2^(v-e))
 	for (int i = 0; i < v - e; ++i) u = A.mul(u, u);
-	for (int side = 0; side < 2; ++side)
+	// y0 = (1 - sqrt 5)/2, y1 = (1 + sqrt 5)/2 = -1/y0 (y0 y1 = -1), so
+	// y1^q = -(y0^q)^-1 (q odd).  For v <= d at most one of them is a 2^d-th
+	// power (z0 = 1 or z0 = -1), for v > d both or neither: a lane needs the
+	// exponentiation y^alpha for a second side only when v > d.
+	const uint32_t y0 = A.mul(A.sub(A.one, s5), inv2), y1 = A.mul(A.add(A.one, s5), inv2);
+	uint64_t alpha;
+	int64_t c;
+	inv_pow2_mod((uint64_t)q, d, alpha, c);
+	const uint32_t z0 = A.pow(y0, q);
+	const uint32_t z1 = A.sub(0u, inv_2sylow<m32, uint32_t>(A, z0, v));
+	const bool ok0 = is_pow2d<m32, uint32_t>(A, z0, v, d), ok1 = is_pow2d<m32, uint32_t>(A, z1, v, d);
+	for (int round = 0; round < 2; ++round)
 	{
-		const uint32_t y = A.mul(side ? A.add(A.one, s5) : A.sub(A.one, s5), inv2);
-		const uint32_t x = root_2d<m32, uint32_t>(A, p, q, v, w, y, d);
-		if (x == 0) continue;
+		const bool act = round ? (ok0 && ok1) : (ok0 || ok1);
+		if (!act) continue;
+		const bool take0 = round == 0 && ok0;
+		const uint32_t y = take0 ? y0 : y1, z = take0 ? z0 : z1;
+		const uint32_t x = root_2d_finish<m32, uint32_t>(A, v, w, A.pow(y, alpha), z, c, d);
 		uint32_t r = x;
 		const uint32_t cnt = 1u << e;
 		for (uint32_t j = 0; j < cnt; ++j)
@@ -326,6 +434,10 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 	uint8_t *d_struck = nullptr;
 	uint32_t *d_list = nullptr, *d_count = nullptr, *d_scount = nullptr;
 	uint2 *d_small = nullptr;
+	sieve_prime *d_sp = nullptr;
+	uint32_t ncoop = 0;
+	const std::vector<sieve_prime> &sp = sieve_primes_neg(&ncoop);
+	if (ncoop > kCoopMax) return fail(CT_ERR_UNSUPPORTED, "sieving prime table larger than the kernel holds");
 	const uint32_t small_cap = 1u << 22;
 	hipStream_t st = nullptr;
 	{
@@ -337,6 +449,11 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 		HIPCHK(hipMalloc((void **)&d_scount, sizeof(uint32_t)));
 		HIPCHK(hipMalloc((void **)&d_small, small_cap * sizeof(uint2)));
 		HIPCHK(hipMemcpy(d_struck, struck, W, hipMemcpyHostToDevice));
+		if (mode <= 0)
+		{
+			HIPCHK(hipMalloc((void **)&d_sp, sp.size() * sizeof(sieve_prime)));
+			HIPCHK(hipMemcpy(d_sp, sp.data(), sp.size() * sizeof(sieve_prime), hipMemcpyHostToDevice));
+		}
 		if (mode >= 0 && kp_max >= kp_min)
 		{
 			for (uint64_t k0 = kp_min; k0 <= kp_max; k0 += KB)
@@ -364,7 +481,8 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 				if (10 * k0 >= P32 + 1) break;
 				HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
 				HIPCHK(hipMemsetAsync(d_scount, 0, sizeof(uint32_t), st));
-				hipLaunchKernelGGL(k_primes_neg, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, d_list, d_count);
+				hipLaunchKernelGGL(k_sieve_neg, dim3((uint32_t)((kc + kSegK - 1) / kSegK)), dim3(256), 0, st, k0, kc, d_sp,
+				                   (uint32_t)sp.size(), ncoop, d_list, d_count);
 				HIPCHK(hipGetLastError());
 				uint32_t cnt = 0, scnt = 0;
 				HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -403,7 +521,7 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 		}
 	}
 done:
-	hipFree(d_struck); hipFree(d_list); hipFree(d_count); hipFree(d_scount); hipFree(d_small);
+	hipFree(d_struck); hipFree(d_list); hipFree(d_count); hipFree(d_scount); hipFree(d_small); hipFree(d_sp);
 	if (st) hipStreamDestroy(st);
 	return rc;
 }

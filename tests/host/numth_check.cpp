@@ -5,7 +5,10 @@
 //  - root_2d returns the same root as the direct construction below (Euler
 //    check by y^((p-1)/2^e), inversions by Fermat's little theorem), and
 //    the root it returns satisfies x^(2^d) = y; 0 exactly when y is no
-//    2^d-th power.
+//    2^d-th power;
+//  - the two sides of the sieve's N- roots: for y1 = -1/y0, y1^q equals
+//    -(y0^q)^-1 by inv_2sylow, is_pow2d on it agrees with the Euler check of
+//    y1, and root_2d_finish from it gives the direct construction's root.
 // Over every prime p = 10k +- 1 below 2^18, the primes p = 1 mod 2^12 below
 // 2^26, the Fermat primes, and random primes below 2^32 and 2^62.
 #include <cstdio>
@@ -95,6 +98,24 @@ static void check_prime(uint64_t p, std::mt19937_64 &rng, int ny)
 			if (!ok && bad++ < 10)
 				std::printf("root p=%llu d=%d y=%llu got=%llu want=%llu\n", (unsigned long long)p, d,
 				            (unsigned long long)y, (unsigned long long)x, (unsigned long long)want);
+			// the other side y1 = -1/y, from z = y^q alone
+			const uint64_t y1 = A.sub(0, A.pow(y, p - 2));
+			const uint64_t z0 = A.pow(y, q);
+			const uint64_t z1 = A.sub(0, inv_2sylow<m64h, uint64_t>(A, z0, v));
+			const int e = (v < d) ? v : d;
+			bool ok1 = (z1 == A.pow(y1, q));
+			const bool pw1 = is_pow2d<m64h, uint64_t>(A, z1, v, d);
+			ok1 = ok1 && (pw1 == (A.pow(y1, (p - 1) >> e) == 1));
+			ok1 = ok1 && (is_pow2d<m64h, uint64_t>(A, z0, v, d) == (A.pow(y, (p - 1) >> e) == 1));
+			if (pw1)
+			{
+				uint64_t alpha;
+				int64_t c;
+				inv_pow2_mod(q, d, alpha, c);
+				ok1 = ok1 && (root_2d_finish<m64h, uint64_t>(A, v, w, A.pow(y1, alpha), z1, c, d) ==
+				              root_direct(A, p, q, v, w, y1, d));
+			}
+			if (!ok1 && bad++ < 10) std::printf("side p=%llu d=%d y=%llu\n", (unsigned long long)p, d, (unsigned long long)y);
 		}
 	}
 }
