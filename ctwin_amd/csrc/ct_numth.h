@@ -12,46 +12,56 @@
 
 #include <cstdint>
 
+#include "ct_arith.h"
+
 namespace ct {
 
 // ---------------------------------------------------------------------------
-// modular arithmetic: 32-bit Montgomery on the device (odd p < 2^32),
+// modular arithmetic: 32-bit Montgomery (odd p < 2^32, device and host),
 // plain 128-bit arithmetic on the host (odd p < 2^63); same interface.
 
-#if defined(__HIPCC__)
 struct m32
 {
 	uint32_t p, q, one, r2;
 
-	__device__ explicit m32(uint32_t p_) : p(p_)
+	CT_HD2 explicit m32(uint32_t p_) : p(p_)
 	{
 		uint32_t x = p;
 		for (int i = 0; i < 5; ++i) x *= 2u - p * x;
 		q = x;
-		one = (uint32_t)((0x100000000ull) % p);
-		r2 = (uint32_t)(((uint64_t)one * one) % p);
+		one = (0u - p) % p;                                  // 2^32 mod p
+		// 2^64 mod p without 64-bit remainders: 2^36 by four doublings, then
+		// three Montgomery squarings (x^2 / 2^32): 2^40, 2^48, 2^64
+		uint32_t t = add(one, one);
+		t = add(t, t); t = add(t, t); t = add(t, t);
+		t = mul(t, t); t = mul(t, t);
+		r2 = mul(t, t);
 	}
-	__device__ uint32_t mul(uint32_t a, uint32_t b) const
+	CT_HD2 uint32_t mul(uint32_t a, uint32_t b) const
 	{
 		const uint64_t t = (uint64_t)a * b;
-		const uint32_t m = (uint32_t)t * q, th = (uint32_t)(t >> 32), mp = __umulhi(m, p);
+		const uint32_t m = (uint32_t)t * q, th = (uint32_t)(t >> 32), mp = mulhi32(m, p);
 		const uint32_t r = th - mp;
 		return (th < mp) ? r + p : r;
 	}
-	__device__ uint32_t add(uint32_t a, uint32_t b) const { const uint32_t s = a + b; return (s < a || s >= p) ? s - p : s; }
-	__device__ uint32_t sub(uint32_t a, uint32_t b) const { return (a >= b) ? a - b : a - b + p; }
-	__device__ uint32_t to_m(uint32_t x) const { return mul(x % p, r2); }
-	__device__ uint32_t from_m(uint32_t x) const { return mul(x, 1u); }
-	__device__ uint32_t neg_one() const { return p - one; }
-	__device__ uint32_t pow(uint32_t a, uint64_t e) const
+	CT_HD2 uint32_t add(uint32_t a, uint32_t b) const { const uint32_t s = a + b; return (s < a || s >= p) ? s - p : s; }
+	CT_HD2 uint32_t sub(uint32_t a, uint32_t b) const { return (a >= b) ? a - b : a - b + p; }
+	CT_HD2 uint32_t to_m(uint32_t x) const { return mul(x % p, r2); }
+	CT_HD2 uint32_t from_m(uint32_t x) const { return mul(x, 1u); }
+	CT_HD2 uint32_t neg_one() const { return p - one; }
+	// Square-and-multiply without branches: each step multiplies by a or by
+	// the Montgomery one.  Exponents are below 2^32 (every caller's is below p).
+	CT_HD2 uint32_t pow(uint32_t a, uint32_t e) const
 	{
 		uint32_t r = one;
-		while (e) { if (e & 1) r = mul(r, a); a = mul(a, a); e >>= 1; }
+		for (; e; e >>= 1)
+		{
+			r = mul(r, (e & 1) ? a : one);
+			a = mul(a, a);
+		}
 		return r;
 	}
 };
-
-#endif
 
 struct m64h
 {
@@ -132,17 +142,18 @@ CT_HD2 U two_sylow_gen(const M &A, U p, U q)
 	}
 }
 
-// a^e1 and a^e2 over one squaring chain.
+// a^e1 and a^e2 over one squaring chain (multiplications by a or by one,
+// without branches).
 template <class M, typename U>
 CT_HD2 void pow_pair(const M &A, U a, uint64_t e1, uint64_t e2, U &r1, U &r2)
 {
 	U x = A.one, y = A.one;
 	while (e1 | e2)
 	{
-		if (e1 & 1) x = A.mul(x, a);
-		if (e2 & 1) y = A.mul(y, a);
+		x = A.mul(x, (e1 & 1) ? a : A.one);
+		y = A.mul(y, (e2 & 1) ? a : A.one);
 		e1 >>= 1; e2 >>= 1;
-		if (e1 | e2) a = A.mul(a, a);
+		a = A.mul(a, a);
 	}
 	r1 = x; r2 = y;
 }

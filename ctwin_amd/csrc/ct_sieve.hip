@@ -249,28 +249,32 @@ __global__ __launch_bounds__(256) void k_sieve_neg(uint64_t k0, uint64_t kcount,
 	for (uint32_t j = t; j < n; j += nt) list[gbase + j] = plist[j];
 }
 
-// The sieving primes, built once on the host.
+// The sieving primes, built once on the host (a function-local static:
+// initialised once even when several host threads call the sieve).
+static std::vector<sieve_prime> build_sieve_primes_neg()
+{
+	std::vector<sieve_prime> v;
+	const uint32_t L = 1u << 16;
+	std::vector<uint8_t> comp(L, 0);
+	for (uint32_t i = 2; i * i < L; ++i)
+		if (!comp[i])
+			for (uint32_t j = i * i; j < L; j += i) comp[j] = 1;
+	for (uint32_t q = 3; q < L; ++q)
+	{
+		if (comp[q] || q == 5) continue;
+		uint32_t r0 = 0;
+		for (uint32_t m = 0; m < 10; ++m)
+			if ((q * m + 1) % 10 == 0) { r0 = (q * m + 1) / 10; break; }   // 10 r0 = 1 mod q
+		v.push_back(sieve_prime{q, r0 | ((q - r0) << 16)});
+	}
+	return v;
+}
+
 static const std::vector<sieve_prime> &sieve_primes_neg(uint32_t *ncoop)
 {
-	static std::vector<sieve_prime> v;
-	static uint32_t nc = 0;
-	if (v.empty())
-	{
-		const uint32_t L = 1u << 16;
-		std::vector<uint8_t> comp(L, 0);
-		for (uint32_t i = 2; i * i < L; ++i)
-			if (!comp[i])
-				for (uint32_t j = i * i; j < L; j += i) comp[j] = 1;
-		for (uint32_t q = 3; q < L; ++q)
-		{
-			if (comp[q] || q == 5) continue;
-			uint32_t r0 = 0;
-			for (uint32_t m = 0; m < 10; ++m)
-				if ((q * m + 1) % 10 == 0) { r0 = (q * m + 1) / 10; break; }   // 10 r0 = 1 mod q
-			v.push_back(sieve_prime{q, r0 | ((q - r0) << 16)});
-			if (q < kCoopQ) ++nc;
-		}
-	}
+	static const std::vector<sieve_prime> v = build_sieve_primes_neg();
+	uint32_t nc = 0;
+	while (nc < v.size() && v[nc].q < kCoopQ) ++nc;       // ascending q
 	*ncoop = nc;
 	return v;
 }

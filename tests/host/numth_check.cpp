@@ -9,6 +9,9 @@
 //  - the two sides of the sieve's N- roots: for y1 = -1/y0, y1^q equals
 //    -(y0^q)^-1 by inv_2sylow, is_pow2d on it agrees with the Euler check of
 //    y1, and root_2d_finish from it gives the direct construction's root.
+//  - the 32-bit Montgomery arithmetic m32 of the sieve kernels (constructor
+//    constants, products, branch-free powers) against 128-bit arithmetic, and
+//    the generator and roots computed with it against the 128-bit ones.
 // Over every prime p = 10k +- 1 below 2^18, the primes p = 1 mod 2^12 below
 // 2^26, the Fermat primes, and random primes below 2^32 and 2^62.
 #include <cstdio>
@@ -58,7 +61,46 @@ static bool is_prime(uint64_t n)
 	return mr_prime<m64h, uint64_t>(m64h(n), n, b, 12);
 }
 
-static long bad = 0, checked = 0, roots = 0;
+static long bad = 0, checked = 0, roots = 0, m32_checked = 0;
+
+// m32 against m64h for an odd prime p < 2^32
+static void check_m32(uint32_t p, std::mt19937_64 &rng)
+{
+	const m32 A(p);
+	const m64h H(p);
+	bool ok = A.one == (uint32_t)((1ull << 32) % p);
+	ok = ok && A.r2 == (uint32_t)(((unsigned __int128)1 << 64) % p);
+	for (int t = 0; t < 20 && ok; ++t)
+	{
+		const uint32_t a = (uint32_t)rng(), b = (uint32_t)rng(), e = (t < 2) ? (uint32_t)t : (uint32_t)rng() >> (t % 32);
+		const uint32_t am = A.to_m(a), bm = A.to_m(b);
+		ok = ok && A.from_m(am) == a % p;
+		ok = ok && A.from_m(A.mul(am, bm)) == H.mul(a % p, b % p);
+		ok = ok && A.from_m(A.add(am, bm)) == H.add(a % p, b % p);
+		ok = ok && A.from_m(A.sub(am, bm)) == H.sub(a % p, b % p);
+		ok = ok && A.from_m(A.pow(am, e)) == H.pow(a % p, e);
+		uint32_t x1, x2;
+		pow_pair<m32, uint32_t>(A, am, e, (uint32_t)rng(), x1, x2);
+		ok = ok && A.from_m(x1) == H.pow(a % p, e);
+	}
+	if (p > 5)
+	{
+		uint32_t q = p - 1; int v = 0;
+		while ((q & 1) == 0) { q >>= 1; ++v; }
+		const uint32_t w = two_sylow_gen<m32, uint32_t>(A, p, q);
+		const uint64_t wh = two_sylow_gen<m64h, uint64_t>(H, p, q);
+		ok = ok && A.from_m(w) == wh;
+		for (int d : {1, 11})
+		{
+			const uint32_t y = (uint32_t)(rng() % (p - 1)) + 1;
+			const uint32_t x = root_2d<m32, uint32_t>(A, p, q, v, w, A.to_m(y), d);
+			const uint64_t xh = root_2d<m64h, uint64_t>(H, p, q, v, wh, y, d);
+			ok = ok && A.from_m(x) == xh;
+		}
+	}
+	++m32_checked;
+	if (!ok && bad++ < 10) std::printf("m32 p=%u\n", p);
+}
 
 static void check_prime(uint64_t p, std::mt19937_64 &rng, int ny)
 {
@@ -143,6 +185,14 @@ int main()
 		const uint64_t p = (rng() >> 2) | 1;
 		if (is_prime(p)) { check_prime(p, rng, 6); ++i; }
 	}
-	std::printf("%ld mismatches in %ld roots (%ld found)\n", bad, checked, roots);
+	for (uint32_t p = 3; p < L; p += 2)
+		if (!comp[p]) check_m32(p, rng);
+	for (uint64_t p : {2147483647ull, 2147483659ull, 4294967291ull, 4294967279ull, 3221225473ull}) check_m32((uint32_t)p, rng);
+	for (int i = 0; i < 20000;)
+	{
+		const uint64_t p = (rng() >> 32) | 1;
+		if (p > 2 && is_prime(p)) { check_m32((uint32_t)p, rng); ++i; }
+	}
+	std::printf("%ld mismatches in %ld roots (%ld found), %ld m32 primes\n", bad, checked, roots, m32_checked);
 	return bad ? 1 : 0;
 }
