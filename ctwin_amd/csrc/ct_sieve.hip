@@ -22,7 +22,10 @@
 // Eratosthenes over k), root kernels that write 1 into
 // struck[b - b_min] (idempotent plain byte stores; no atomics needed).  Primes
 // below 2^16 on the - side strike too many b per root for one thread: their
-// (p, root) pairs are queued and struck by one workgroup each.
+// (p, root) pairs are queued and struck by one workgroup each.  A - prime with
+// more than 32 roots (2^e, e = min(v2(p-1), m-1)) queues its run of roots for a
+// wavefront of its own, so that no lane walks up to 2^(m-1) roots while its
+// wavefront waits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -281,9 +284,33 @@ static const std::vector<sieve_prime> &sieve_primes_neg(uint32_t *ncoop)
 
 constexpr uint32_t kSmallP = 1u << 16;
 
+// A run of roots x u^j (j < 2^e) of one prime, in Montgomery form.
+struct root_run { uint32_t p, x, u, e; };
+
+constexpr int kRunInline = 5;              // runs of up to 32 roots stay in their lane (8: 0.6 ms slower in all)
+
+// Strikes (or queues, for p below kSmallP) the cnt roots r, r u, r u^2, ...
+__device__ __forceinline__ void strike_roots(const m32 &A, uint32_t p, uint32_t r, uint32_t u, uint32_t cnt, uint8_t *struck,
+                                             uint64_t b_min, uint64_t b_max, uint2 *small, uint32_t *small_count,
+                                             uint32_t small_cap)
+{
+	for (uint32_t j = 0; j < cnt; ++j)
+	{
+		const uint32_t b = A.from_m(r);
+		if (p < kSmallP)
+		{
+			const uint32_t slot = atomicAdd(small_count, 1u);
+			if (slot < small_cap) small[slot] = make_uint2(p, b);
+		}
+		else strike_one(struck, b, p, b_min, b_max);
+		r = A.mul(r, u);
+	}
+}
+
 // one thread per prime: sqrt(5), y = (1 +/- sqrt 5)/2, all 2^(m-1)-th roots of y
 __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, uint64_t b_min, uint64_t b_max,
-                            uint8_t *struck, uint2 *small, uint32_t *small_count, uint32_t small_cap)
+                            uint8_t *struck, uint2 *small, uint32_t *small_count, uint32_t small_cap,
+                            root_run *big, uint32_t *big_count, uint32_t big_cap)
 {
 	const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= *count) return;
@@ -316,19 +343,33 @@ __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, 
 		const bool take0 = round == 0 && ok0;
 		const uint32_t y = take0 ? y0 : y1, z = take0 ? z0 : z1;
 		const uint32_t x = root_2d_finish<m32, uint32_t>(A, v, w, A.pow(y, alpha), z, c, d);
-		uint32_t r = x;
-		const uint32_t cnt = 1u << e;
-		for (uint32_t j = 0; j < cnt; ++j)
-		{
-			const uint32_t b = A.from_m(r);
-			if (p < kSmallP)
-			{
-				const uint32_t slot = atomicAdd(small_count, 1u);
-				if (slot < small_cap) small[slot] = make_uint2(p, b);
-			}
-			else strike_one(struck, b, p, b_min, b_max);
-			r = A.mul(r, u);
+		if (e > kRunInline)
+		{   // many roots: a wavefront of k_roots_run shares them, unless the queue is full
+			const uint32_t slot = atomicAdd(big_count, 1u);
+			if (slot < big_cap) { big[slot] = root_run{p, x, u, (uint32_t)e}; continue; }
 		}
+		strike_roots(A, p, x, u, 1u << e, struck, b_min, b_max, small, small_count, small_cap);
+	}
+}
+
+// The queued runs of 2^e > 2^kRunInline roots, one wavefront each: lane l
+// takes the roots x u^j for j in [l 2^e / 64, (l + 1) 2^e / 64), or the single
+// root j = l of a run shorter than 64.  A grid-stride loop covers every queued
+// run.
+__global__ __launch_bounds__(64) void k_roots_run(const root_run *big, const uint32_t *big_count, uint32_t big_cap,
+                                                  uint64_t b_min, uint64_t b_max, uint8_t *struck, uint2 *small,
+                                                  uint32_t *small_count, uint32_t small_cap)
+{
+	const uint32_t n = min(*big_count, big_cap);
+	for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
+	{
+		const root_run R = big[i];
+		const m32 A(R.p);
+		const uint32_t cnt = 1u << R.e;
+		const uint32_t per = (cnt >= 64) ? cnt / 64 : (threadIdx.x < cnt ? 1u : 0u);
+		const uint32_t j0 = (cnt >= 64) ? threadIdx.x * per : threadIdx.x;
+		const uint32_t r = A.mul(R.x, A.pow(R.u, j0));
+		strike_roots(A, R.p, r, R.u, per, struck, b_min, b_max, small, small_count, small_cap);
 	}
 }
 
@@ -438,6 +479,8 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 	uint8_t *d_struck = nullptr;
 	uint32_t *d_list = nullptr, *d_count = nullptr, *d_scount = nullptr;
 	uint2 *d_small = nullptr;
+	root_run *d_big = nullptr;
+	const uint32_t big_cap = 1u << 16;
 	sieve_prime *d_sp = nullptr;
 	uint32_t ncoop = 0;
 	const std::vector<sieve_prime> &sp = sieve_primes_neg(&ncoop);
@@ -450,11 +493,12 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 		HIPCHK(hipMalloc((void **)&d_struck, W));
 		HIPCHK(hipMalloc((void **)&d_list, 2 * KB * sizeof(uint32_t)));
 		HIPCHK(hipMalloc((void **)&d_count, sizeof(uint32_t)));
-		HIPCHK(hipMalloc((void **)&d_scount, sizeof(uint32_t)));
+		HIPCHK(hipMalloc((void **)&d_scount, 2 * sizeof(uint32_t)));      // small pairs, queued runs
 		HIPCHK(hipMalloc((void **)&d_small, small_cap * sizeof(uint2)));
 		HIPCHK(hipMemcpy(d_struck, struck, W, hipMemcpyHostToDevice));
 		if (mode <= 0)
 		{
+			HIPCHK(hipMalloc((void **)&d_big, big_cap * sizeof(root_run)));
 			HIPCHK(hipMalloc((void **)&d_sp, sp.size() * sizeof(sieve_prime)));
 			HIPCHK(hipMemcpy(d_sp, sp.data(), sp.size() * sizeof(sieve_prime), hipMemcpyHostToDevice));
 		}
@@ -484,7 +528,7 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 				const uint64_t kc = std::min(KB, kn_max - k0 + 1);
 				if (10 * k0 >= P32 + 1) break;
 				HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
-				HIPCHK(hipMemsetAsync(d_scount, 0, sizeof(uint32_t), st));
+				HIPCHK(hipMemsetAsync(d_scount, 0, 2 * sizeof(uint32_t), st));
 				hipLaunchKernelGGL(k_sieve_neg, dim3((uint32_t)((kc + kSegK - 1) / kSegK)), dim3(256), 0, st, k0, kc, d_sp,
 				                   (uint32_t)sp.size(), ncoop, d_list, d_count);
 				HIPCHK(hipGetLastError());
@@ -494,6 +538,9 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 				if (cnt)
 				{
 					hipLaunchKernelGGL(k_roots_neg, dim3((cnt + 127) / 128), dim3(128), 0, st, d_list, d_count, m, b_min, b_max,
+					                   d_struck, d_small, d_scount, small_cap, d_big, d_scount + 1, big_cap);
+					HIPCHK(hipGetLastError());
+					hipLaunchKernelGGL(k_roots_run, dim3(1024), dim3(64), 0, st, d_big, d_scount + 1, big_cap, b_min, b_max,
 					                   d_struck, d_small, d_scount, small_cap);
 					HIPCHK(hipGetLastError());
 					HIPCHK(hipMemcpyAsync(&scnt, d_scount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -525,7 +572,7 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 		}
 	}
 done:
-	hipFree(d_struck); hipFree(d_list); hipFree(d_count); hipFree(d_scount); hipFree(d_small); hipFree(d_sp);
+	hipFree(d_struck); hipFree(d_list); hipFree(d_count); hipFree(d_scount); hipFree(d_small); hipFree(d_big); hipFree(d_sp);
 	if (st) hipStreamDestroy(st);
 	return rc;
 }
