@@ -39,6 +39,17 @@ def test_gpu_sieve_sides_match_oracle(mode):
     assert np.array_equal(got, want)
 
 
+def test_gpu_sieve_neg_range_off_segment_bounds():
+    """An N- p range [12345671, 6e7]: k starts at 1234567, inside a sieve
+    segment, and ends inside another."""
+    m, b_min, b_max = 12, (1 << 20) + 777, (1 << 20) + 777 + 60000
+    print("\n[case] sieve N- offset p range", flush=True)
+    got = ct.sieve_window(m, b_min, b_max, 10 ** 6, 60000000, mode=-1, p_min_neg=12345671)
+    want = O.sieve_window(m, b_min, b_max, 10 ** 6, 60000000, mode=-1, p_min_neg=12345671)
+    assert want.sum() > 0
+    assert np.array_equal(got, want)
+
+
 def test_gpu_sieve_accumulates_into_struck():
     """struck is read and updated, as the reference's _bsieve."""
     m, b_min, b_max = 10, 5000, 9000
