@@ -114,16 +114,16 @@ constexpr uint64_t kQrMask64[17] = {qr_mask(3), qr_mask(5), qr_mask(7), qr_mask(
                                     qr_mask(19), qr_mask(23), qr_mask(29), qr_mask(31), qr_mask(37), qr_mask(41),
                                     qr_mask(43), qr_mask(47), qr_mask(53), qr_mask(59), qr_mask(61)};
 
-// Generator of the 2-Sylow subgroup (order 2^v) of an odd prime p: w = c^q,
-// p - 1 = 2^v q, c the least quadratic non-residue.  c is prime (a product of
-// residues is a residue).  Below 64 its character comes from p mod c:
-// (2/p) = -1 iff p = 3, 5 mod 8, and for odd c != p quadratic reciprocity gives
-// (c/p) = (p/c) (-1)^((c-1)/2 (p-1)/2).  Past 61 the Euler criterion decides.
+// The least quadratic non-residue c of an odd prime p.  c is prime (a
+// product of residues is a residue).  Below 64 its character comes from p
+// mod c: (2/p) = -1 iff p = 3, 5 mod 8, and for odd c != p quadratic
+// reciprocity gives (c/p) = (p/c) (-1)^((c-1)/2 (p-1)/2).  Past 61 the Euler
+// criterion decides.
 template <class M, typename U>
-CT_HD2 U two_sylow_gen(const M &A, U p, U q)
+CT_HD2 uint32_t least_nonresidue(const M &A, U p)
 {
 	const uint32_t p8 = (uint32_t)(p & 7u);
-	if (p8 == 3 || p8 == 5) return A.pow(A.to_m((U)2), q);
+	if (p8 == 3 || p8 == 5) return 2;
 	const bool p3 = (p & 3u) == 3;
 #pragma unroll
 	for (int i = 0; i < 17; ++i)
@@ -133,13 +133,18 @@ CT_HD2 U two_sylow_gen(const M &A, U p, U q)
 		if (r == 0) continue;                              // c = p
 		const bool res = (kQrMask64[i] >> r) & 1u;          // (p/c) = 1
 		const bool flip = ((c & 3u) == 3) && p3;
-		if (res == flip) return A.pow(A.to_m((U)c), q);
+		if (res == flip) return c;
 	}
 	for (uint32_t c = 67;; ++c)
-	{
-		const U cm = A.to_m((U)c);
-		if (A.pow(cm, (p - 1) / 2) == A.neg_one()) return A.pow(cm, q);
-	}
+		if (A.pow(A.to_m((U)c), (p - 1) / 2) == A.neg_one()) return c;
+}
+
+// Generator of the 2-Sylow subgroup (order 2^v) of an odd prime p: w = c^q,
+// p - 1 = 2^v q, c the least quadratic non-residue.
+template <class M, typename U>
+CT_HD2 U two_sylow_gen(const M &A, U p, U q)
+{
+	return A.pow(A.to_m((U)least_nonresidue(A, p)), q);
 }
 
 // a^e1 and a^e2 over one squaring chain (multiplications by a or by one,
@@ -156,6 +161,37 @@ CT_HD2 void pow_pair(const M &A, U a, uint64_t e1, uint64_t e2, U &r1, U &r2)
 		a = A.mul(a, a);
 	}
 	r1 = x; r2 = y;
+}
+
+// a^ea and b^eb in one loop: two independent chains, for latency.
+template <class M, typename U>
+CT_HD2 void pow_dual(const M &A, U a, uint64_t ea, U b, uint64_t eb, U &ra, U &rb)
+{
+	U x = A.one, y = A.one;
+	while (ea | eb)
+	{
+		x = A.mul(x, (ea & 1) ? a : A.one);
+		y = A.mul(y, (eb & 1) ? b : A.one);
+		ea >>= 1; eb >>= 1;
+		a = A.mul(a, a); b = A.mul(b, b);
+	}
+	ra = x; rb = y;
+}
+
+// a^e1, a^e2 and b^e2 in one loop (a's chain shared by the first two).
+template <class M, typename U>
+CT_HD2 void pow_tri(const M &A, U a, U b, uint64_t e1, uint64_t e2, U &r1, U &r2, U &r3)
+{
+	U x = A.one, y = A.one, z = A.one;
+	while (e1 | e2)
+	{
+		x = A.mul(x, (e1 & 1) ? a : A.one);
+		y = A.mul(y, (e2 & 1) ? a : A.one);
+		z = A.mul(z, (e2 & 1) ? b : A.one);
+		e1 >>= 1; e2 >>= 1;
+		a = A.mul(a, a); b = A.mul(b, b);
+	}
+	r1 = x; r2 = y; r3 = z;
 }
 
 // alpha = 2^-d mod q (q odd) by d halvings modulo q: a/2 for even a,

@@ -318,22 +318,31 @@ __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, 
 	const m32 A(p);
 	uint32_t q = p - 1; int v = 0;
 	while ((q & 1) == 0) { q >>= 1; ++v; }
-	const uint32_t w = two_sylow_gen<m32, uint32_t>(A, p, q);
-	const uint32_t s5 = root_2d<m32, uint32_t>(A, p, q, v, w, A.to_m(5u), 1);
-	if (s5 == 0) return;
+	// The exponentiations run two or three chains per loop: the kernel is
+	// bound by the latency of dependent products, not by their number.
+	// w = c^q and t = 5^((q-1)/2) together; sqrt 5 = 5^alpha (2^d-th root for
+	// d = 1, alpha = (q + 1)/2, c = 1) from x1 = 5 t, z = t x1 = 5^q.
+	const uint32_t five = A.to_m(5u);
+	uint32_t w, t5;
+	pow_dual<m32, uint32_t>(A, A.to_m(least_nonresidue(A, p)), q, five, (q - 1) / 2, w, t5);
+	if (q == 1) return;                                    // p = 2^v + 1: not of the form 10k +- 1
+	const uint32_t x15 = A.mul(t5, five), z5 = A.mul(t5, x15);
+	if (!is_pow2d<m32, uint32_t>(A, z5, v, 1)) return;
+	const uint32_t s5 = root_2d_finish<m32, uint32_t>(A, v, w, x15, z5, 1, 1);
 	const uint32_t inv2 = A.to_m((p + 1) / 2);
 	const int d = m - 1, e = (v < d) ? v : d;
 	uint32_t u = w;                                        // primitive 2^e-th root: w^(2^(v-e))
 	for (int i = 0; i < v - e; ++i) u = A.mul(u, u);
 	// y0 = (1 - sqrt 5)/2, y1 = (1 + sqrt 5)/2 = -1/y0 (y0 y1 = -1), so
 	// y1^q = -(y0^q)^-1 (q odd).  For v <= d at most one of them is a 2^d-th
-	// power (z0 = 1 or z0 = -1), for v > d both or neither: a lane needs the
-	// exponentiation y^alpha for a second side only when v > d.
+	// power (z0 = 1 or z0 = -1), for v > d both or neither.  z0 = y0^q,
+	// y0^alpha and y1^alpha come from one loop.
 	const uint32_t y0 = A.mul(A.sub(A.one, s5), inv2), y1 = A.mul(A.add(A.one, s5), inv2);
 	uint64_t alpha;
 	int64_t c;
 	inv_pow2_mod((uint64_t)q, d, alpha, c);
-	const uint32_t z0 = A.pow(y0, q);
+	uint32_t z0, xa0, xa1;
+	pow_tri<m32, uint32_t>(A, y0, y1, q, alpha, z0, xa0, xa1);
 	const uint32_t z1 = A.sub(0u, inv_2sylow<m32, uint32_t>(A, z0, v));
 	const bool ok0 = is_pow2d<m32, uint32_t>(A, z0, v, d), ok1 = is_pow2d<m32, uint32_t>(A, z1, v, d);
 	for (int round = 0; round < 2; ++round)
@@ -341,8 +350,7 @@ __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, 
 		const bool act = round ? (ok0 && ok1) : (ok0 || ok1);
 		if (!act) continue;
 		const bool take0 = round == 0 && ok0;
-		const uint32_t y = take0 ? y0 : y1, z = take0 ? z0 : z1;
-		const uint32_t x = root_2d_finish<m32, uint32_t>(A, v, w, A.pow(y, alpha), z, c, d);
+		const uint32_t x = root_2d_finish<m32, uint32_t>(A, v, w, take0 ? xa0 : xa1, take0 ? z0 : z1, c, d);
 		if (e > kRunInline)
 		{   // many roots: a wavefront of k_roots_run shares them, unless the queue is full
 			const uint32_t slot = atomicAdd(big_count, 1u);

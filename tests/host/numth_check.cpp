@@ -10,7 +10,8 @@
 //    -(y0^q)^-1 by inv_2sylow, is_pow2d on it agrees with the Euler check of
 //    y1, and root_2d_finish from it gives the direct construction's root.
 //  - the 32-bit Montgomery arithmetic m32 of the sieve kernels (constructor
-//    constants, products, branch-free powers) against 128-bit arithmetic, and
+//    constants, products, branch-free powers, the two- and three-chain power
+//    loops) against 128-bit arithmetic, and
 //    the generator and roots computed with it against the 128-bit ones.
 // Over every prime p = 10k +- 1 below 2^18, the primes p = 1 mod 2^12 below
 // 2^26, the Fermat primes, and random primes below 2^32 and 2^62.
@@ -82,6 +83,12 @@ static void check_m32(uint32_t p, std::mt19937_64 &rng)
 		uint32_t x1, x2;
 		pow_pair<m32, uint32_t>(A, am, e, (uint32_t)rng(), x1, x2);
 		ok = ok && A.from_m(x1) == H.pow(a % p, e);
+		const uint32_t e2 = (uint32_t)rng() >> (t % 32);
+		uint32_t d1, d2, t1, t2, t3;
+		pow_dual<m32, uint32_t>(A, am, e, bm, e2, d1, d2);
+		ok = ok && A.from_m(d1) == H.pow(a % p, e) && A.from_m(d2) == H.pow(b % p, e2);
+		pow_tri<m32, uint32_t>(A, am, bm, e, e2, t1, t2, t3);
+		ok = ok && A.from_m(t1) == H.pow(a % p, e) && A.from_m(t2) == H.pow(a % p, e2) && A.from_m(t3) == H.pow(b % p, e2);
 	}
 	if (p > 5)
 	{
