@@ -22,7 +22,8 @@
 // Eratosthenes over k), root kernels that write 1 into
 // struck[b - b_min] (idempotent plain byte stores; no atomics needed).  Primes
 // below 2^16 on the - side strike too many b per root for one thread: their
-// (p, root) pairs are queued and struck by one workgroup each.  A - prime with
+// (p, root) pairs are queued and struck by one workgroup each (+ primes below
+// 2^20 too).  A - prime with
 // more than 32 roots (2^e, e = min(v2(p-1), m-1)) queues its run of roots for a
 // wavefront of its own, so that no lane walks up to 2^(m-1) roots while its
 // wavefront waits.
@@ -98,8 +99,17 @@ __global__ void k_primes_pos(uint64_t k0, uint64_t kcount, int m, uint32_t *list
 	append(keep, p, list, count);
 }
 
+// Primes below these strike too many b per root for one thread (about
+// 2^24 / p in a 2^24 window): their (p, root) pairs are queued for
+// k_strike_small, one workgroup per pair.  On the + side every prime has 2^m
+// roots, and one workgroup walking them for p = 3 2^12 + 1 would keep its CU
+// busy for milliseconds (each byte store touches 64 cache lines).
+constexpr uint32_t kSmallP = 1u << 16;
+constexpr uint32_t kSmallPosP = 1u << 20;
+
 // one workgroup per prime: find c, then 256 threads walk g = c^6
-__global__ void k_roots_pos(const uint32_t *list, const uint32_t *count, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck)
+__global__ void k_roots_pos(const uint32_t *list, const uint32_t *count, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck,
+                            uint2 *small, uint32_t *small_count, uint32_t small_cap)
 {
 	const uint32_t idx = blockIdx.x;
 	if (idx >= *count) return;
@@ -132,8 +142,17 @@ __global__ void k_roots_pos(const uint32_t *list, const uint32_t *count, int m, 
 	uint32_t cur = A.pow(g, t0);
 	for (uint32_t t = t0; t < t1; ++t)
 	{
-		strike_one(struck, A.from_m(A.mul(c, cur)), p, b_min, b_max);
-		strike_one(struck, A.from_m(A.mul(c5, cur)), p, b_min, b_max);
+		const uint32_t b1 = A.from_m(A.mul(c, cur)), b5 = A.from_m(A.mul(c5, cur));
+		if (p < kSmallPosP)
+		{
+			const uint32_t slot = atomicAdd(small_count, 2u);
+			if (slot + 1 < small_cap) { small[slot] = make_uint2(p, b1); small[slot + 1] = make_uint2(p, b5); }
+		}
+		else
+		{
+			strike_one(struck, b1, p, b_min, b_max);
+			strike_one(struck, b5, p, b_min, b_max);
+		}
 		cur = A.mul(cur, g);
 	}
 }
@@ -282,7 +301,6 @@ static const std::vector<sieve_prime> &sieve_primes_neg(uint32_t *ncoop)
 	return v;
 }
 
-constexpr uint32_t kSmallP = 1u << 16;
 
 // A run of roots x u^j (j < 2^e) of one prime, in Montgomery form.
 struct root_run { uint32_t p, x, u, e; };
@@ -517,6 +535,7 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 				const uint64_t kc = std::min(KB, kp_max - k0 + 1);
 				if (3 * (k0 << m) + 1 >= P32) break;
 				HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
+				HIPCHK(hipMemsetAsync(d_scount, 0, 2 * sizeof(uint32_t), st));
 				hipLaunchKernelGGL(k_primes_pos, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, m, d_list, d_count);
 				HIPCHK(hipGetLastError());
 				uint32_t cnt = 0;
@@ -524,8 +543,18 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 				HIPCHK(hipStreamSynchronize(st));
 				if (cnt)
 				{
-					hipLaunchKernelGGL(k_roots_pos, dim3(cnt), dim3(256), 0, st, d_list, d_count, m, b_min, b_max, d_struck);
+					hipLaunchKernelGGL(k_roots_pos, dim3(cnt), dim3(256), 0, st, d_list, d_count, m, b_min, b_max, d_struck,
+					                   d_small, d_scount, small_cap);
 					HIPCHK(hipGetLastError());
+					uint32_t scnt = 0;
+					HIPCHK(hipMemcpyAsync(&scnt, d_scount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+					HIPCHK(hipStreamSynchronize(st));
+					if (scnt > small_cap) { rc = fail(CT_ERR_UNSUPPORTED, "too many small-prime roots"); goto done; }
+					if (scnt)
+					{
+						hipLaunchKernelGGL(k_strike_small, dim3(scnt), dim3(256), 0, st, d_small, d_scount, b_min, b_max, d_struck);
+						HIPCHK(hipGetLastError());
+					}
 				}
 			}
 		}
