@@ -163,9 +163,10 @@ CT_HD2 void pow_pair(const M &A, U a, uint64_t e1, uint64_t e2, U &r1, U &r2)
 	r1 = x; r2 = y;
 }
 
-// a^ea and b^eb in one loop: two independent chains, for latency.
-template <class M, typename U>
-CT_HD2 void pow_dual(const M &A, U a, uint64_t ea, U b, uint64_t eb, U &ra, U &rb)
+// a^ea and b^eb in one loop: two independent chains, for latency.  E is the
+// exponent type (32 bits suffice for p < 2^32).
+template <class M, typename U, typename E = uint64_t>
+CT_HD2 void pow_dual(const M &A, U a, E ea, U b, E eb, U &ra, U &rb)
 {
 	U x = A.one, y = A.one;
 	while (ea | eb)
@@ -179,8 +180,8 @@ CT_HD2 void pow_dual(const M &A, U a, uint64_t ea, U b, uint64_t eb, U &ra, U &r
 }
 
 // a^e1, a^e2 and b^e2 in one loop (a's chain shared by the first two).
-template <class M, typename U>
-CT_HD2 void pow_tri(const M &A, U a, U b, uint64_t e1, uint64_t e2, U &r1, U &r2, U &r3)
+template <class M, typename U, typename E = uint64_t>
+CT_HD2 void pow_tri(const M &A, U a, U b, E e1, E e2, U &r1, U &r2, U &r3)
 {
 	U x = A.one, y = A.one, z = A.one;
 	while (e1 | e2)

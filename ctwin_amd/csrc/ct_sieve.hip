@@ -342,7 +342,7 @@ __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, 
 	// d = 1, alpha = (q + 1)/2, c = 1) from x1 = 5 t, z = t x1 = 5^q.
 	const uint32_t five = A.to_m(5u);
 	uint32_t w, t5;
-	pow_dual<m32, uint32_t>(A, A.to_m(least_nonresidue(A, p)), q, five, (q - 1) / 2, w, t5);
+	pow_dual<m32, uint32_t, uint32_t>(A, A.to_m(least_nonresidue(A, p)), q, five, (q - 1) / 2, w, t5);
 	if (q == 1) return;                                    // p = 2^v + 1: not of the form 10k +- 1
 	const uint32_t x15 = A.mul(t5, five), z5 = A.mul(t5, x15);
 	if (!is_pow2d<m32, uint32_t>(A, z5, v, 1)) return;
@@ -360,7 +360,7 @@ __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, 
 	int64_t c;
 	inv_pow2_mod((uint64_t)q, d, alpha, c);
 	uint32_t z0, xa0, xa1;
-	pow_tri<m32, uint32_t>(A, y0, y1, q, alpha, z0, xa0, xa1);
+	pow_tri<m32, uint32_t, uint32_t>(A, y0, y1, q, (uint32_t)alpha, z0, xa0, xa1);      // alpha < q
 	const uint32_t z1 = A.sub(0u, inv_2sylow<m32, uint32_t>(A, z0, v));
 	const bool ok0 = is_pow2d<m32, uint32_t>(A, z0, v, d), ok1 = is_pow2d<m32, uint32_t>(A, z1, v, d);
 	for (int round = 0; round < 2; ++round)
