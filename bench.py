@@ -2,8 +2,9 @@
 
 Workload (the batched configuration): 256 candidates N = b^n - b^(n/2) + 1,
 n = 2^14 digits, b uniform in [2^18, 2^19) from a fixed seed, run through the
-base-2 PRP loop (x <- x^2 * 2^bit) in one persistent resident launch; a step is
-one squaring of all 256 candidates.  Beside it: the plain one-candidate engine
+base-2 PRP loop (x <- x^2 * 2^bit, each lane from its own top exponent bit) in
+one persistent launch of the autotuned engine; a step is one squaring of all
+256 candidates, and every lane is checked against GMP after the timed run.  Beside it: the plain one-candidate engine
 at n = 2^14 (batched / plain ratio) and the single large candidate at n = 2^21
 (b = 65536 + 2u), with its HBM roofline, and the oracle timed on the host CPU.
 
@@ -38,23 +39,72 @@ def bytes_alg(n):
     return 24 * n * (1 if n <= (1 << 13) else 2)
 
 
+BATCH_FIXTURE = os.path.join(ROOT, "tests", "golden", "batched_n2e14_x256.json")
+
+
+def batch_parity(ct, t, bs, done, i0):
+    """After the timed run (`done` squarings from x = 1): continue untimed to
+    the next GMP checkpoint of tests/golden/batched_n2e14_x256.json and compare
+    RES64 and x mod three 31-bit primes of every lane's canonical residue."""
+    if not os.path.exists(BATCH_FIXTURE):
+        return {"checked": False, "reason": "fixture missing"}
+    fx = json.load(open(BATCH_FIXTURE))
+    if [L["b"] for L in fx["lanes"]] != list(bs):
+        return {"checked": False, "reason": "fixture drawn for other b"}
+    later = [tc for tc in fx["t"] if tc >= done]
+    if not later:
+        return {"checked": False, "reason": f"no GMP checkpoint at or after {done} squarings"}
+    tc = later[0]
+    if tc > done:
+        t.run(i0 - done, i0 - tc + 1)
+    r64, mods = ct.residue_checks(t.get_int(0), bs, fx["q"])
+    k = fx["t"].index(tc)
+    bad = [l for l, L in enumerate(fx["lanes"])
+           if "%016X" % r64[l] != L["aligned_res64"][k] or mods[l] != L["aligned_mod_q"][k]]
+    return {"checked": True, "lanes": len(bs), "squarings": tc, "timed_squarings": done,
+            "against": "GMP (tests/golden/batched_n2e14_x256.json): RES64 and x mod 3 primes",
+            "lanes_differing": len(bad), "bit_exact": not bad}
+
+
 def run_batched(ct, steps, warmup):
+    """256 lanes, each from its own top exponent bit: lane l's exponent is
+    shifted left so that its top bit sits at the common index i0, so every
+    lane carries a non-trivial residue from the first squaring (in a lockstep
+    batch from a common i0, lanes with a shorter exponent square x = 1 for up
+    to thousands of steps).  After `steps` timed squarings lane l holds
+    2^(top `steps` bits of E_l) mod N_l."""
     from ctwin_amd.exponent import exponent
     ln, n, lanes = 14, 1 << 14, 256
     rng = random.Random(SEED)
     bs = [rng.randrange(1 << 18, 1 << 19) for _ in range(lanes)]
     E = [exponent(b, n) for b in bs]
-    i0 = max(e.bit_length() for e in E) - 1
+    top = max(e.bit_length() for e in E)
+    i0 = top - 1
+    aligned = [e << (top - e.bit_length()) for e in E]
     # configuration chosen by the autotuner (timed before this run, outside the timed region)
     t = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
-    t.set_b(bs); t.set_exponents(E); t.set(1)
-    t.run(i0, i0 - warmup + 1)
+    t.set_b(bs); t.set_exponents(aligned); t.set(1)
+    t.run(i0, i0 - warmup + 1)                       # warm-up, then the state restarts from x = 1
+    t.set(1)
     t.timer_start()
-    t.run(i0 - warmup, i0 - warmup - steps + 1)
+    t.run(i0, i0 - steps + 1)
     ms = t.timer_stop()
     kind, tile = t.kind, t.tuning()
+    parity = batch_parity(ct, t, bs, steps, i0)
     t.close()
-    return lanes * steps / (ms / 1e3), ms, i0, bs, kind, tile
+    return lanes * steps / (ms / 1e3), ms, i0, bs, kind, tile, parity
+
+
+def prp_work(i0):
+    """Squaring-equivalents of one PRP test with Gerbicz-Li (cyclo.h:252-377)
+    whose exponent's top bit is i0: the loop's i0 + 1 squarings, L = 2^(1 +
+    floor(log2 i0)/2) products d(t) x (one product costs about two squarings:
+    two forward transforms), the check's B + 11 squarings (B = ceil(i0 / L),
+    the top bit of the B-bit chunk sum is below B + log2 L + 1) and its two
+    products."""
+    L = 2 << ((i0.bit_length() - 1) // 2)
+    B = (i0 - 1) // L + 1
+    return (i0 + 1) + 2 * L + (B + 11) + 4
 
 
 def run_plain(ct, ln, b, steps, warmup, graphs=True):
@@ -211,7 +261,7 @@ def main():
         return
 
     if args.profile:
-        run_batched(ct, 50, 1)
+        run_batched(ct, 50, 1)[:0]
         b0 = random.Random(SEED).randrange(1 << 18, 1 << 19)
         run_plain(ct, 14, b0, 20, 1, graphs=False)
         run_plain(ct, 21, 65536 + 2 * random.Random(SEED).randrange(1 << 14), 20, 1, graphs=False)
@@ -222,8 +272,9 @@ def main():
         print("[bench] profile run done (copy kernel moves 8388608 bytes each way)", flush=True)
         return
 
-    sq_s, ms, i0, bs, kind, tuning = run_batched(ct, args.steps, args.warmup)
-    print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps)", flush=True)
+    sq_s, ms, i0, bs, kind, tuning, parity = run_batched(ct, args.steps, args.warmup)
+    print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps); "
+          f"parity {parity}", flush=True)
     plain_sq_s, plain_ms, plain_cfg = run_plain(ct, 14, bs[0], args.plain_steps, 5, graphs=not args.no_graphs)
     print(f"[bench] plain n=2^14 x1: {plain_sq_s:.4g} squarings/s", flush=True)
     big_b = 65536 + 2 * random.Random(SEED).randrange(1 << 14)
@@ -237,7 +288,7 @@ def main():
           f"p99 {served['latency_p99_ms']:.1f} ms", flush=True)
 
     n = 1 << 14
-    per_cand = i0 + 2 * int(math.isqrt(i0)) + 64       # PRP loop + Gerbicz-Li tail (approx.)
+    per_cand = prp_work(i0)
     valu_achieved = mul32_alg(n) * sq_s / 1e12
     big_n = 1 << 21
     hbm_achieved = bytes_alg(big_n) * big_sq_s / 1e9
@@ -255,11 +306,13 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded b)",
         "config": {"workload": "batched: 256 candidates b^(2^14) - b^(2^13) + 1, b in [2^18, 2^19), "
-                               "base-2 PRP loop, one persistent resident launch",
+                               "base-2 PRP loop from each lane's own top exponent bit (every residue "
+                               "non-trivial), one persistent launch of the autotuned engine",
                    "log2_n": 14, "candidates": 256, "seed": SEED,
                    "engine": "resident" if kind == ct.KIND_RESIDENT else "multipass",
                    "autotuner_ms_per_step": dict(zip(TUNE_NAMES, tuning[3]))},
         "candidates_per_hour": 3600.0 * sq_s / per_cand,
+        "parity": parity,
         "plain": {"squarings_per_s": plain_sq_s, "candidates_per_hour": 3600.0 * plain_sq_s / per_cand,
                   "engine": "plain multipass, one candidate per launch sequence", "tuned": plain_cfg},
         "batched_vs_plain": sq_s / plain_sq_s,

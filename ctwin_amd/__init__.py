@@ -35,7 +35,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
            "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split",
-           "ct_engine_set_fused"]
+           "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free"]
 
 INTERRUPTED = 1
 
@@ -113,6 +113,9 @@ def lib():
                                  np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
             "ct_engine_set_column_split": [vp, i32],
             "ct_engine_set_fused": [vp, i32],
+            "ct_prp_begin": [vp, u32p, c.POINTER(vp), c.POINTER(i32), c.POINTER(i32)],
+            "ct_prp_finish": [vp, vp, ip, u64p, ip, c.c_void_p],
+            "ct_prp_job_free": [vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -294,6 +297,71 @@ def prp_check(t, bs, want_residue=False):
         _check(rc)
     t.b = bs.copy()
     return prm.astype(bool), r64, gl.astype(bool), (res.reshape(t.lanes, t.n) if res is not None else None), rc
+
+
+class PrpJob:
+    """ct_prp_check split at its loop: begin() sets b and the exponent bits
+    and gives (i0, gl_block); the caller runs x = 1, d(t) = 1 and the loop
+    (Transform.run(i0, 0, gl_block), in pieces); finish() returns what
+    prp_check returns."""
+
+    def __init__(self, t, bs):
+        self.t = t
+        self.bs = np.ascontiguousarray(bs, np.uint32)
+        h = ctypes.c_void_p(); i0 = ctypes.c_int(); B = ctypes.c_int()
+        _check(lib().ct_prp_begin(t.h, self.bs, ctypes.byref(h), ctypes.byref(i0), ctypes.byref(B)))
+        self.h, self.i0, self.gl_block = h, i0.value, B.value
+        t.b = self.bs.copy()
+
+    def finish(self, want_residue=False):
+        t = self.t
+        prm = np.zeros(t.lanes, np.int32); r64 = np.zeros(t.lanes, np.uint64); gl = np.zeros(t.lanes, np.int32)
+        res = np.zeros(t.lanes * t.n, np.uint32) if want_residue else None
+        rc = lib().ct_prp_finish(t.h, self.h, prm, r64, gl, res.ctypes.data_as(ctypes.c_void_p) if res is not None else None)
+        if rc not in (0, -6, -7):
+            _check(rc)
+        return prm.astype(bool), r64, gl.astype(bool), (res.reshape(t.lanes, t.n) if res is not None else None), rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ct_prp_job_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def residue_checks(canon, bs, qs):
+    """Checksums of values x_l = sum_j canon[l, j] b_l^j given by canonical
+    digits: RES64 (x mod 2^64, cyclo.h:51-57) and x mod q for each q < 2^31
+    (x mod q sees every digit; RES64 sees only the low ones when b is even).
+    Returns (res64[lanes], mods[lanes][len(qs)]) as Python ints."""
+    d = np.ascontiguousarray(canon, np.uint64).reshape(len(bs), -1)
+    lanes, n = d.shape
+    b = np.asarray(bs, np.uint64).reshape(-1, 1)
+    pw = np.empty((lanes, n), np.uint64); pw[:, 0] = 1
+    if n > 1:
+        pw[:, 1:] = b
+        np.cumprod(pw, axis=1, out=pw)                       # b^j mod 2^64 (wrapping)
+    r64 = [int(v) for v in np.sum(d * pw, axis=1, dtype=np.uint64)]
+    mods = [[0] * len(qs) for _ in range(lanes)]
+    for k, q in enumerate(qs):
+        qq = np.uint64(q)
+        pq = np.ones((lanes, n), np.uint64)
+        cur = b[:, 0] % qq
+        step = 1
+        while step < n:                                       # pq[:, j] = b^j mod q by doubling
+            w = min(step, n - step)
+            pq[:, step:step + w] = (pq[:, :w] * cur[:, None]) % qq
+            cur = (cur * cur) % qq
+            step *= 2
+        prod = (d * pq) % qq                                  # digits < 2^31, so products < 2^62
+        for l in range(lanes):
+            s = 0
+            for j in range(0, n, 1 << 15):
+                s += int(np.sum(prod[l, j:j + (1 << 15)], dtype=np.uint64))
+            mods[l][k] = s % q
+    return r64, mods
 
 
 class Server:

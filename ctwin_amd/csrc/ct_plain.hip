@@ -823,8 +823,14 @@ int plain_alloc(ct_engine *e)
 	return CT_OK;
 }
 
+// Destroys the cached executables.  Replays of them may still be queued on
+// the engine's stream (plain_steps launches a run as consecutive replays and
+// may come here between two of them), so the stream drains first: an
+// executable is destroyed only after its last replay has completed.
 static void drop_graphs(ct_engine *e)
 {
+	if (e->graphs.empty()) return;
+	if (e->st) (void)hipStreamSynchronize(e->st);
 	for (auto &g : e->graphs) hipGraphExecDestroy(g.exec);
 	e->graphs.clear();
 }

@@ -129,3 +129,38 @@ int ct_prp_check(ct_engine *e, const uint32_t *b, int *prm, uint64_t *res64, int
 	if ((rc = ct_engine_set(e, 1)) || (rc = ct_engine_copy(e, 1, 0)) || (rc = ct_engine_run(e, J.i0, 0, J.B))) return rc;
 	return prp_finish(e, J, prm, res64, gl_ok, residue);
 }
+
+// ct_prp_check split at its loop, for callers that drive the loop themselves
+// (checkpointing, interruption, or a test that alters the state mid-run).
+struct ct_prp_job
+{
+	prp_job J;
+	bool finished = false;
+};
+
+int ct_prp_begin(ct_engine *e, const uint32_t *b, ct_prp_job **out, int *i0, int *gl_block)
+{
+	if (!e || !b || !out) return fail(CT_ERR_ARG, "bad argument");
+	*out = nullptr;
+	ct_prp_job *job = new ct_prp_job();
+	const int rc = prp_begin(e, b, job->J);
+	if (rc) { delete job; return rc; }
+	if (i0) *i0 = job->J.i0;
+	if (gl_block) *gl_block = job->J.B;
+	*out = job;
+	return CT_OK;
+}
+
+int ct_prp_finish(ct_engine *e, ct_prp_job *job, int *prm, uint64_t *res64, int *gl_ok, uint32_t *residue)
+{
+	if (!e || !job || !prm || !res64 || !gl_ok) return fail(CT_ERR_ARG, "bad argument");
+	if (job->finished) return fail(CT_ERR_ARG, "the PRP job has already been finished");
+	job->finished = true;          // prp_finish consumes the exponents (their B-bit chunk sums replace them)
+	return prp_finish(e, job->J, prm, res64, gl_ok, residue);
+}
+
+int ct_prp_job_free(ct_prp_job *job)
+{
+	delete job;
+	return CT_OK;
+}

@@ -494,9 +494,12 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 	const uint64_t kn_min = p_min_neg / 10;
 	uint64_t kn_max = p_max_neg / 10;
 	while (10 * kn_max + 1 < p_max_neg) ++kn_max;
-	if (mode >= 0 && 3 * (kp_max << m) + 1 >= P32 && 3 * ((kp_max - 1) << m) + 1 >= P32)
+	// an empty k range (kp_max < kp_min, or kn_max == 0 when p_max_neg <= 1)
+	// processes nothing, as check_pos / check_neg do for it
+	if (mode >= 0 && kp_max >= kp_min && kp_max > 0 && 3 * (kp_max << m) + 1 >= P32 &&
+	    3 * ((kp_max - 1) << m) + 1 >= P32)
 		return fail(CT_ERR_UNSUPPORTED, "+ side: p beyond 2^32");
-	if (mode <= 0 && 10 * kn_max - 1 >= P32 && 10 * (kn_max - 1) + 1 >= P32)
+	if (mode <= 0 && kn_max >= kn_min && kn_max > 0 && 10 * kn_max - 1 >= P32 && 10 * (kn_max - 1) + 1 >= P32)
 		return fail(CT_ERR_UNSUPPORTED, "- side: p beyond 2^32");
 
 	const size_t W = (size_t)(b_max - b_min + 1);

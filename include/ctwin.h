@@ -157,6 +157,21 @@ int ct_engine_run(ct_engine *e, int i_hi, int i_lo, int gl_block);
  * any lane fails verification, CT_ERR_COMPUTE if any residue is 0. */
 int ct_prp_check(ct_engine *e, const uint32_t *b, int *prm, uint64_t *res64, int *gl_ok, uint32_t *residue);
 
+/* ct_prp_check split at its loop, as cyclo::check is split by its
+ * checkpoints (cyclo.h:256-276 before the loop, :310-377 after it):
+ * ct_prp_begin sets b and uploads the exponent bits of E = N - 1 (GMP,
+ * cyclo.h:258-271), returning the top bit index i0 and the Gerbicz-Li block B
+ * (cyclo.h:272).  The caller then sets register 0 := 1, copies it to
+ * register 1 (d(t) = 1) and runs ct_engine_run(e, i0, 0, B), in as many
+ * pieces as it likes.  ct_prp_finish is the tail of cyclo::check (the last
+ * d(t) product, the Gerbicz-Li verification run, RES64 and the verdict) with
+ * the outputs and status of ct_prp_check; a job is finished once
+ * (CT_ERR_ARG the second time) and freed with ct_prp_job_free. */
+typedef struct ct_prp_job ct_prp_job;
+int ct_prp_begin(ct_engine *e, const uint32_t *b, ct_prp_job **job, int *i0, int *gl_block);
+int ct_prp_finish(ct_engine *e, ct_prp_job *job, int *prm, uint64_t *res64, int *gl_ok, uint32_t *residue);
+int ct_prp_job_free(ct_prp_job *job);
+
 /* Wall time of the device work issued since the last call (ms), measured with
  * HIP events on the engine's stream. */
 int ct_engine_timer_start(ct_engine *e);

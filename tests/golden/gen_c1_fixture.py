@@ -1,0 +1,47 @@
+"""Writes tests/golden/c1_n2e10.json: GMP Fermat residues of config 1, the
+plain PRP of the candidate pair N+- = b^1024 - b^512 +- 1 (n = 2^10 digits),
+b an even integer uniform in [2^19, 2^20) drawn from seed 20261203, for a
+batch of 64 such b (the reference's 64 lanes, cyclo/src/engine.h:19; b[0] is
+the plain one-candidate case): RES64 of base^(N-1) mod N for bases 2 and 3
+and both members.
+
+Test infrastructure (the oracle's mpz_powm), run in the build container:
+    python tests/golden/gen_c1_fixture.py
+"""
+import json
+import os
+import random
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import oracle_lib as O                                    # noqa: E402
+
+SEED = 20261203
+LN, LANES = 10, 64
+
+
+def batch_bs():
+    rng = random.Random(SEED + 1)
+    return [2 * rng.randrange(1 << 18, 1 << 19) for _ in range(LANES)]
+
+
+def main():
+    n = 1 << LN
+    rows = []
+    for b in batch_bs():
+        row = {"b": b}
+        for sign, name in ((1, "plus"), (-1, "minus")):
+            N = b ** n - b ** (n // 2) + sign
+            for base in (2, 3):
+                r = O.powm(base, N - 1, N)
+                row[f"{name}_base{base}"] = {"res64": "%016X" % (r % (1 << 64)), "prp": r == 1}
+        rows.append(row)
+    path = os.path.join(HERE, "c1_n2e10.json")
+    with open(path, "w") as f:
+        json.dump({"log2_n": LN, "seed": SEED + 1, "lanes": rows}, f, separators=(",", ":"))
+    print(f"wrote {path}: {len(rows)} b")
+
+
+if __name__ == "__main__":
+    main()
