@@ -211,25 +211,108 @@ def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30, graphs=
             "b_source": "GPU sieve survivors (both members, p < 2^26) of 2^16-wide windows"}
 
 
-def cpu_baseline(budget_s=12.0):
-    """The oracle (plain-C restatement, one core) on a bounded sample of the
-    same workload: squarings of one n = 2^14 candidate from the batch."""
+def _cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(omp)) if omp and omp.isdigit() else avail
+    return model, os.cpu_count(), avail, max(1, threads)
+
+
+def _cpu_rate(C, eng, x, bits, words, i0, threads, lanes, budget_s, chunk):
+    """Squarings/s of the CPU comparator: runs of `chunk` PRP-loop steps until
+    budget_s has elapsed or the exponent window (bits i0 .. 0) is used up
+    (at least one run)."""
+    done, t0 = 0, time.perf_counter()
+    while True:
+        eng.steps(x, bits, words, i0 - done, chunk, threads)
+        done += chunk
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or done + chunk > i0 + 1:
+            return lanes * done / dt, lanes * done, dt
+
+
+def cpu_baseline(ct, budget_s=5.0):
+    """CPU baselines on the GPU host (the cpu_baseline leg):
+    - the C++ comparator (oracle/cpu_port: the same two-prime NTT, Garner and
+      chunk carry as the engine, OpenMP) on bounded samples of the batched
+      workload (n = 2^14, lanes of the bench's batch) and of the single large
+      candidate (n = 2^21), one thread and every thread this process may use;
+    - the reference's own CPU window sieve (oracle/_ref/ctsieve_ref, compiled
+      from sieve/CPU/ctsieve.cpp) on config 2's b window with p < 2^28, beside
+      the GPU sieve on the same sample (same survivors required)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import hashlib
+    import subprocess
+    import tempfile
     import numpy as np
-    import oracle_lib as O
+    import cpu_port as C
+    from ctwin_amd.exponent import exponent, top_bits
+    model, nproc, avail, threads = _cpu_info()
     rng = random.Random(SEED)
-    b = rng.randrange(1 << 18, 1 << 19)
+    bs = [rng.randrange(1 << 18, 1 << 19) for _ in range(256)]
     n = 1 << 14
-    e = O.Engine(n, b, 16)
-    d = np.array([random.Random(1).randrange(-b + 1, b) for _ in range(n)], np.int32)
-    count = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        e.square_dup(d, count & 1)
-        count += 1
-    dt = time.perf_counter() - t0
-    return {"value": count / dt, "unit": "squarings/s", "cores": 1, "kind": "port",
-            "sample": f"{count} squarings of one n=2^14 candidate (b={b}) by the oracle in {dt:.1f} s"}
+    out = {"unit": "squarings/s", "kind": "port", "cpu_model": model, "nproc": nproc, "cpus_available": avail}
+    rates = {}
+    for th in (1, threads):
+        lanes = 2 * th
+        E = [exponent(b, n) for b in bs[:lanes]]
+        top = max(e.bit_length() for e in E)
+        bits, words = C.pack_bits([e << (top - e.bit_length()) for e in E], lanes)
+        eng = C.CpuEngine(14, bs[:lanes])
+        x = np.zeros((lanes, n), np.int32); x[:, 0] = 1
+        r, cnt, dt = _cpu_rate(C, eng, x, bits, words, top - 1, th, lanes, budget_s, 4)
+        rates[th] = r
+        out.setdefault("n2e14", {})[f"threads_{th}"] = {"squarings_per_s": r, "sample": f"{cnt} squarings ({lanes} lanes) in {dt:.1f} s"}
+        eng.close()
+    big_b = 65536 + 2 * random.Random(SEED).randrange(1 << 14)
+    w, sh = top_bits(big_b, 1 << 21, 4096)
+    bits, words = C.pack_bits([w], 1)
+    for th in (1, threads):
+        eng = C.CpuEngine(21, [big_b])
+        x = np.zeros((1, 1 << 21), np.int32); x[0, 0] = 1
+        r, cnt, dt = _cpu_rate(C, eng, x, bits, words, 4095, th, 1, budget_s, 1)
+        out.setdefault("n2e21", {})[f"threads_{th}"] = {"squarings_per_s": r, "sample": f"{cnt} squarings in {dt:.1f} s"}
+        eng.close()
+    out["value"] = rates[threads]
+    out["cores"] = threads
+    out["sample"] = (f"C++ comparator, n=2^14 batch lanes, {threads} OpenMP threads over lanes, "
+                     f"{budget_s:.0f} s per point")
+    # the reference's CPU sieve on a bounded sample, and the GPU sieve beside it
+    ref = os.path.join(ROOT, "oracle", "_ref", "ctsieve_ref")
+    m, b_min, b_max, pm = 12, 1 << 20, (1 << 20) + (1 << 24) - 1, 1 << 28
+    sv = {"kind": "reference", "sample": f"m=12, b in [2^20, 2^20 + 2^24), p < 2^28 both sides, 1 thread "
+                                         "(sieve/CPU/ctsieve.cpp, its own build)"}
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as d:
+            f = os.path.join(d, "surv.txt")
+            t0 = time.perf_counter()
+            rc = subprocess.run([ref, str(m), str(b_min), str(b_max), str(pm), str(pm), f],
+                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL).returncode
+            dt = time.perf_counter() - t0
+            ref_text = open(f).read() if rc == 0 else None
+        t0 = time.perf_counter()
+        st = ct.sieve_window(m, b_min, b_max, pm, pm)
+        gdt = time.perf_counter() - t0
+        gpu_text = "".join(f"{int(b)}\n" for b in ct.survivors(st, b_min))
+        sv.update({"reference_seconds": dt, "gpu_seconds": gdt, "reference_rc": rc,
+                   "survivors": gpu_text.count("\n"),
+                   "gpu_equals_reference": ref_text is not None and
+                   hashlib.sha256(ref_text.encode()).digest() == hashlib.sha256(gpu_text.encode()).digest()})
+    else:
+        sv["skipped"] = "oracle/_ref/ctsieve_ref not built"
+    out["sieve"] = sv
+    out["sieve_config2_reference_s_build_container"] = {
+        "seconds": 177.5, "where": "the build container (Xeon 2.7 GHz, 1 thread), not the GPU host",
+        "sample": "the full config 2 (p < 2^32 both sides)"}
+    return out
 
 
 def main():
@@ -328,7 +411,7 @@ def main():
                      "kernel": "k_res_steps<14>", "mul32_alg_per_squaring": mul32_alg(n)},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(ct)
     print(json.dumps(out), flush=True)
 
 
