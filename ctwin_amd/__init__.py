@@ -35,7 +35,8 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
            "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split",
-           "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free"]
+           "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free",
+           "ct_sieve_factors_noalloc"]
 
 INTERRUPTED = 1
 

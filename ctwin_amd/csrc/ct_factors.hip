@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ctwin.h"
@@ -133,6 +134,29 @@ __global__ void k_froots_neg(const uint64_t *list, const uint32_t *count, int m,
 	}
 }
 
+// On-device re-verification of emitted pairs (the check of saveFactors,
+// ctsieve.h:270-289): p divides b^(2^m) - b^(2^(m-1)) + 1 (mode +1) or - 1.
+__global__ void k_fverify(const uint64_t *pairs, const unsigned long long *ocount, uint64_t cap, int m, int mode,
+                          uint32_t *bad)
+{
+	const uint64_t n = min((uint64_t)*ocount, cap);
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+	{
+		const uint64_t p = pairs[2 * i], b = pairs[2 * i + 1];
+		const m64d A(p);
+		uint64_t x = A.to_m(b);
+		for (int k = 0; k < m - 1; ++k) x = A.mul(x, x);
+		const uint64_t r = A.from_m(A.sub(A.mul(x, x), x));
+		if (r != ((mode > 0) ? p - 1 : 1)) atomicAdd(bad, 1u);
+	}
+}
+
+// clears n words (a kernel, not hipMemsetAsync)
+__global__ void k_fzero(uint32_t *w, uint32_t n)
+{
+	for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) w[i] = 0;
+}
+
 // host re-verification (saveFactors, ctsieve.h:270-289)
 static bool host_divides(uint64_t p, uint64_t b, int m, int mode)
 {
@@ -203,4 +227,89 @@ done:
 	hipFree(d_list); hipFree(d_out); hipFree(d_count); hipFree(d_ocount);
 	if (st) hipStreamDestroy(st);
 	return rc;
+}
+
+// ---------------------------------------------------------------------------
+// ct_sieve_factors without allocation: the same kernels with their scratch in
+// a file-scope device array.  Layout of `scratch` (per device):
+//   [0, 8 KB_NA)  the prime list of one block of k (at most 2 primes per k),
+//   then four 32-bit words {prime count, mismatches, unused, unused} and the
+//   64-bit emitted-pair counter.
+// Every call on one device shares the array; calls that overlap on a device race.
+
+namespace {
+
+constexpr uint64_t KB_NA = 1ull << 20;                                  // k values per block
+constexpr size_t kNaList = 2 * KB_NA * sizeof(uint64_t);                // 16 MiB
+constexpr size_t kNaScratch = kNaList + 256;
+
+__device__ __align__(256) uint8_t scratch[kNaScratch];
+
+std::mutex g_na_mu;
+void *g_na_ptr[64];                                                     // per device, from hipGetSymbolAddress
+
+int na_scratch(int device, uint8_t **out)
+{
+	if (device < 0 || device >= 64) return fail(CT_ERR_ARG, "device ordinal out of range");
+	std::lock_guard<std::mutex> g(g_na_mu);
+	if (!g_na_ptr[device])
+	{
+		void *ptr = nullptr;
+		const hipError_t h = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (h != hipSuccess) return hip_fail(h, "hipGetSymbolAddress(scratch)");
+		g_na_ptr[device] = ptr;
+	}
+	*out = static_cast<uint8_t *>(g_na_ptr[device]);
+	return CT_OK;
+}
+
+}   // namespace
+
+#define NACHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) return hip_fail(_e, #call); } while (0)
+
+int ct_sieve_factors_noalloc(int device, int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap,
+                             uint64_t *count, void *stream)
+{
+	if (!count || (cap && !pairs)) return fail(CT_ERR_ARG, "bad argument");
+	if (m < 8 || m > 24 || mode == 0) return fail(CT_ERR_ARG, "m in [8, 24], mode +1 or -1");
+	if (p_max < p_min || p_max >= (1ull << 63)) return fail(CT_ERR_UNSUPPORTED, "p range must lie below 2^63");
+	const uint64_t k_lo = (mode > 0) ? (p_min >> m) / 3 : p_min / 10;
+	const uint64_t k_hi = (mode > 0) ? (p_max >> m) / 3 + 1 : p_max / 10 + 1;
+	NACHK(hipSetDevice(device));
+	uint8_t *base = nullptr;
+	int rc = na_scratch(device, &base);
+	if (rc) return rc;
+	hipStream_t st = static_cast<hipStream_t>(stream);
+	uint64_t *d_list = reinterpret_cast<uint64_t *>(base);
+	uint32_t *d_words = reinterpret_cast<uint32_t *>(base + kNaList);      // {count, bad, -, -}
+	uint32_t *d_count = d_words, *d_bad = d_words + 1;
+	unsigned long long *d_ocount = reinterpret_cast<unsigned long long *>(base + kNaList + 16);
+	hipLaunchKernelGGL(k_fzero, dim3(1), dim3(64), 0, st, d_words, 6u);     // the four words and the 64-bit counter
+	NACHK(hipGetLastError());
+	for (uint64_t k0 = k_lo; k0 <= k_hi; k0 += KB_NA)
+	{
+		const uint64_t kc = std::min(KB_NA, k_hi - k0 + 1);                 // at most 2 kc primes: the list holds them
+		hipLaunchKernelGGL(k_fzero, dim3(1), dim3(64), 0, st, d_count, 1u);
+		hipLaunchKernelGGL(k_fprimes, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, m, mode, p_min, p_max,
+		                   d_list, d_count);
+		NACHK(hipGetLastError());
+		uint32_t cnt = 0;
+		NACHK(hipMemcpyAsync(&cnt, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+		NACHK(hipStreamSynchronize(st));
+		if (!cnt) continue;
+		if (mode > 0) hipLaunchKernelGGL(k_froots_pos, dim3(cnt), dim3(256), 0, st, d_list, d_count, m, pairs, d_ocount, cap);
+		else hipLaunchKernelGGL(k_froots_neg, dim3((cnt + 127) / 128), dim3(128), 0, st, d_list, d_count, m, pairs, d_ocount, cap);
+		NACHK(hipGetLastError());
+	}
+	hipLaunchKernelGGL(k_fverify, dim3(1024), dim3(256), 0, st, pairs, d_ocount, cap, m, mode, d_bad);
+	NACHK(hipGetLastError());
+	unsigned long long oc = 0;
+	uint32_t bad = 0;
+	NACHK(hipMemcpyAsync(&oc, d_ocount, sizeof(oc), hipMemcpyDeviceToHost, st));
+	NACHK(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
+	NACHK(hipStreamSynchronize(st));
+	*count = oc;
+	if (bad) return fail(CT_ERR_COMPUTE, "emitted pair does not divide");
+	if (oc > cap) return fail(CT_ERR_NOMEM, "pairs buffer too small (count holds the size needed)");
+	return CT_OK;
 }

@@ -252,6 +252,23 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 int ct_sieve_factors(int device, int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap,
                      uint64_t *count);
 
+/* ct_sieve_factors that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (serving loops that must not
+ * allocate, harnesses that forbid hipMalloc); the call itself reads counters
+ * back and synchronises its stream, so it is not for use inside a graph
+ * capture.  pairs is a
+ * DEVICE buffer of 2 * cap uint64 words owned by the caller; the same pairs
+ * as ct_sieve_factors (unordered), re-verified on the device instead of on
+ * the host; *count = number of pairs, CT_ERR_NOMEM when it exceeds cap (the
+ * first cap pairs are written).  stream: a hipStream_t (NULL: the default
+ * stream); the call returns after the stream has finished its work.  Scratch:
+ * a fixed 16 MiB + 256 B device array per device (the prime list of one block
+ * of 2^20 k values and the counters), kept until the process ends; the p range
+ * is processed block by block, so every range below 2^63 fits it.  Every call
+ * on a device shares that array: calls that overlap on one device race. */
+int ct_sieve_factors_noalloc(int device, int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs,
+                             uint64_t cap, uint64_t *count, void *stream);
+
 /* ---------------- files and the batch driver ---------------- */
 
 /* CRC-32 (IEEE 802.3, reflected) continued over len more bytes; the files
