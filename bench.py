@@ -25,12 +25,66 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_T = 39.3          # 256 CU x 64 lanes x 2.4 GHz, 32-bit ops (mul_hi/mul_lo full rate on gfx950)
 SEED = 20261203
 TUNE_NAMES = ("resident", "multipass_w16", "multipass_w32", "multipass_w16_split", "multipass_w32_split",
-              "multipass_w16_fused", "multipass_w32_fused")
+              "multipass_w16_fused", "multipass_w32_fused", "multipass_w16_fused_split", "multipass_w32_fused_split",
+              "multipass_best_rows_in_sequence")
 
 
-def mul32_alg(n):
-    """Algorithmic 32-bit multiplies per squaring: 9 n (log2 n + 4)."""
-    return 9 * n * (math.log2(n) + 4)
+def mul32_alg(n, primes=3):
+    """Algorithmic 32-bit multiplies per squaring: per prime n (log2 n + 2.5)
+    modular products for the transforms, plus the CRT (three primes: 3 n
+    products, two primes: n), 3 mul32 per product -- 9 n (log2 n + 4) with
+    three primes, 6 n (log2 n + 3) with two."""
+    if primes == 3:
+        return 9 * n * (math.log2(n) + 4)
+    return 6 * n * (math.log2(n) + 3)
+
+
+# Measured HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# on `bench.py --profile`, raw KiB scaled by what the same counters read for a
+# copy kernel of known size: FETCH x 1.994, WRITE x 1.000).  The resident
+# kernel reads its state and tables once per launch whatever the number of
+# squarings; the n = 2^21 figures are per squaring (row pass + fused column pass).
+TRAFFIC_RESIDENT_LAUNCH = (24434 * 1.994 + 45056) * 1024     # k_res_steps<14, 2>, 256 lanes
+TRAFFIC_N2E21_SQUARING = (33170 * 1.994 + 16384 + 8633 * 1.994 + 16904) * 1024   # k_rowB + k_colCA2
+VALU_WAVE_INSTR_PER_STEP = 963507776 / 51                      # k_res_steps<14, 2>: 256 lanes, one squaring
+
+
+def run_c1(ct):
+    """Config 1: the full Fermat PRP test with Gerbicz-Li of the candidate
+    pair N+- = b^1024 - b^512 +- 1, b even, uniform in [2^19, 2^20) (seed +
+    1): base 3 (the configuration's) for both members and base 2 (the
+    reference's) for N+, as a batch of 64 (the reference's lane count) and as
+    the plain one-candidate engine; RES64 of every test against GMP
+    (tests/golden/c1_n2e10.json)."""
+    ln, lanes = 10, 64
+    rng = random.Random(SEED + 1)
+    bs = [2 * rng.randrange(1 << 18, 1 << 19) for _ in range(lanes)]
+    fx_path = os.path.join(ROOT, "tests", "golden", "c1_n2e10.json")
+    fx = json.load(open(fx_path)) if os.path.exists(fx_path) else None
+    if fx and [L["b"] for L in fx["lanes"]] != bs:
+        fx = None
+    out = {"log2_n": ln, "b_range": [1 << 19, 1 << 20], "seed": SEED + 1}
+    bad = 0
+    checked = 0
+    for name, sign, base in (("plus_base3", 1, 3), ("minus_base3", -1, 3), ("plus_base2", 1, 2)):
+        for count in (lanes, 1):
+            t = ct.Transform(ln, count, ct.KIND_AUTO, csize=0, sign=sign)
+            t.set_base(base)
+            ct.prp_check(t, bs[:count])                         # warm-up (tuned engine, first launches)
+            t0 = time.perf_counter()
+            prm, r64, gl, _, rc = ct.prp_check(t, bs[:count])
+            dt = time.perf_counter() - t0
+            t.close()
+            key = f"{name}_{'batched64' if count == lanes else 'plain1'}"
+            out[key] = {"seconds_per_batch": dt, "candidates_per_hour": 3600.0 * count / dt,
+                        "gerbicz_ok": bool(gl.all()), "status": int(rc)}
+            if fx:
+                for l in range(count):
+                    checked += 1
+                    bad += ("%016X" % int(r64[l])) != fx["lanes"][l][name]["res64"]
+    out["parity"] = {"checked": fx is not None, "tests": checked, "differing": bad,
+                     "against": "GMP (tests/golden/c1_n2e10.json): RES64 of base^(N-1) mod N"}
+    return out
 
 
 def bytes_alg(n):
@@ -66,7 +120,7 @@ def batch_parity(ct, t, bs, done, i0):
             "lanes_differing": len(bad), "bit_exact": not bad}
 
 
-def run_batched(ct, steps, warmup):
+def run_batched(ct, steps, warmup, csize=0):
     """256 lanes, each from its own top exponent bit: lane l's exponent is
     shifted left so that its top bit sits at the common index i0, so every
     lane carries a non-trivial residue from the first squaring (in a lockstep
@@ -82,7 +136,7 @@ def run_batched(ct, steps, warmup):
     i0 = top - 1
     aligned = [e << (top - e.bit_length()) for e in E]
     # configuration chosen by the autotuner (timed before this run, outside the timed region)
-    t = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
+    t = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=csize)
     t.set_b(bs); t.set_exponents(aligned); t.set(1)
     t.run(i0, i0 - warmup + 1)                       # warm-up, then the state restarts from x = 1
     t.set(1)
@@ -107,15 +161,19 @@ def prp_work(i0):
     return (i0 + 1) + 2 * L + (B + 11) + 4
 
 
-def run_plain(ct, ln, b, steps, warmup, graphs=True):
+def run_plain(ct, ln, b, steps, warmup, graphs=True, config=None):
     """Plain multipass engine, one candidate: `warmup` untimed squarings, then
     `steps` timed ones.  The warm-up also runs one run of the timed length, so
     that the captured graph of that length exists (a long PRP test replays it
-    millions of times)."""
+    millions of times).  config = (tile, split, fused) instead of the autotuner."""
     from ctwin_amd.exponent import window_value
     n = 1 << ln
     Ew, i0 = window_value(b, n, 2 * steps + warmup + 8)
-    t = ct.Transform(ln, 1, ct.KIND_PLAIN, csize=0)     # multipass, tile and pass split from the autotuner
+    if config is None:
+        t = ct.Transform(ln, 1, ct.KIND_PLAIN, csize=0)     # multipass, tile and pass split from the autotuner
+    else:
+        t = ct.Transform(ln, 1, ct.KIND_PLAIN)
+        t.set_column_tile(config[0]); t.set_column_split(config[1]); t.set_fused(config[2])
     t.set_graphs(graphs)
     t.set_b([b]); t.set_exponents([Ew]); t.set(1)
     t.run(i0, i0 - warmup + 1)
@@ -124,10 +182,11 @@ def run_plain(ct, ln, b, steps, warmup, graphs=True):
     t.timer_start()
     t.run(i, i - steps + 1)
     ms = t.timer_stop()
-    w, split, fused, _ = t.tuning()
+    w, split, fused, _, row_seq = t.tuning()
     t.read_state(0)                 # the read checks the fused pass's carry exchanges
     t.close()
-    return steps / (ms / 1e3), ms, {"column_tile": w, "split_pass_a": split, "fused_pass_ca": fused}
+    return steps / (ms / 1e3), ms, {"column_tile": w, "split_pass_a": split, "fused_pass_ca": fused,
+                                    "row_primes_in_sequence": row_seq}
 
 
 def run_sieve_config2(ct):
@@ -203,10 +262,17 @@ def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30, graphs=
     st = s.stats()
     s.close()
     L = np.array(sorted(lat.values()))
+    per_class = {}
+    for k in range(12, 21):
+        v = np.array([lat[r[0]] for r in reqs if r[1] == k])
+        if len(v):
+            per_class[f"2^{k}"] = {"requests": int(len(v)), "p50_ms": 1e3 * float(np.percentile(v, 50)),
+                                   "p99_ms": 1e3 * float(np.percentile(v, 99))}
     return {"requests": requests, "squarings_per_request": steps, "arrival_rate_per_s": rate,
             "wall_s": wall, "squarings_per_s": (st["squarings"] - sq0) / wall,
             "requests_per_s": requests / wall,
             "latency_p50_ms": 1e3 * float(np.percentile(L, 50)), "latency_p99_ms": 1e3 * float(np.percentile(L, 99)),
+            "latency_by_class": per_class,
             "hbm_budget_bytes": budget, "log2_n_range": [12, 20], "b_bits_range": [16, 22],
             "b_source": "GPU sieve survivors (both members, p < 2^26) of 2^16-wide windows"}
 
@@ -329,25 +395,28 @@ def main():
     ap.add_argument("--sieve-only", action="store_true",
                     help="run only the config-2 window sieve and print its JSON object (for profiling the sieve)")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for hardware counters: 50 batched steps in one launch, 20 plain "
-                         "squarings at n=2^14 and n=2^21, and 3 copies of an 8 MiB register "
-                         "(calibration of FETCH_SIZE / WRITE_SIZE)")
+                    help="short run for hardware counters, without graphs or autotuning: 50 batched steps "
+                         "in one resident launch, 20 plain squarings at n=2^14 (column tile 16) and n=2^21 "
+                         "(tile 32), fused column pass with the primes in workgroup halves, and 3 copies of "
+                         "an 8 MiB register (calibration of FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the candidates shard one queue per GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import ctwin_amd as ct
+    if args.no_graphs or args.profile:
+        ct.set_graphs_default(False)          # no graph anywhere in the process, the autotuner's included
 
     if args.sieve_only:
         print(json.dumps({"sieve_config2": run_sieve_config2(ct)}), flush=True)
         return
 
     if args.profile:
-        run_batched(ct, 50, 1)[:0]
+        run_batched(ct, 50, 1, csize=16)[:0]
         b0 = random.Random(SEED).randrange(1 << 18, 1 << 19)
-        run_plain(ct, 14, b0, 20, 1, graphs=False)
-        run_plain(ct, 21, 65536 + 2 * random.Random(SEED).randrange(1 << 14), 20, 1, graphs=False)
+        run_plain(ct, 14, b0, 20, 1, graphs=False, config=(16, True, 1))
+        run_plain(ct, 21, 65536 + 2 * random.Random(SEED).randrange(1 << 14), 20, 1, graphs=False, config=(32, True, 1))
         t = ct.Transform(21, 1, ct.KIND_PLAIN)
         for _ in range(3):
             t.copy(1, 0)
@@ -369,10 +438,17 @@ def main():
     served = run_served(ct, graphs=not args.no_graphs)
     print(f"[bench] served: {served['squarings_per_s']:.4g} squarings/s, p50 {served['latency_p50_ms']:.1f} ms, "
           f"p99 {served['latency_p99_ms']:.1f} ms", flush=True)
+    served_light = run_served(ct, requests=48, rate=40.0, graphs=not args.no_graphs)
+    print(f"[bench] served below saturation: {served_light['squarings_per_s']:.4g} squarings/s, "
+          f"p50 {served_light['latency_p50_ms']:.1f} ms, p99 {served_light['latency_p99_ms']:.1f} ms", flush=True)
 
+    c1 = run_c1(ct)
+    print(f"[bench] config 1: {c1['plus_base3_batched64']['candidates_per_hour']:.4g} N+ tests/hour (64 lanes), "
+          f"parity {c1['parity']}", flush=True)
     n = 1 << 14
     per_cand = prp_work(i0)
-    valu_achieved = mul32_alg(n) * sq_s / 1e12
+    valu_achieved = mul32_alg(n, 2) * sq_s / 1e12
+    valu_issue = VALU_WAVE_INSTR_PER_STEP * 4 / (1024 * 2.4e9) / (ms / 1e3 / args.steps)
     big_n = 1 << 21
     hbm_achieved = bytes_alg(big_n) * big_sq_s / 1e9
     out = {
@@ -402,13 +478,20 @@ def main():
         "single_large": {"log2_n": 21, "b": big_b, "squarings_per_s": big_sq_s, "ms_per_squaring": big_ms / args.big_steps,
                          "tuned": big_cfg,
                          "roofline": {"bound": "hbm", "achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                      "frac": hbm_achieved / HBM_PEAK_GBS, "traffic": None,
+                                      "frac": hbm_achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_N2E21_SQUARING,
                                       "bytes_alg_per_squaring": bytes_alg(big_n)}},
         "sieve_config2": sieve,
         "served": served,
+        "served_below_saturation": served_light,
+        "config1": c1,
         "roofline": {"bound": "valu", "achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "Tmul32/s",
-                     "frac": valu_achieved / VALU_PEAK_T, "traffic": None,
-                     "kernel": "k_res_steps<14>", "mul32_alg_per_squaring": mul32_alg(n)},
+                     "frac": valu_achieved / VALU_PEAK_T, "traffic": TRAFFIC_RESIDENT_LAUNCH,
+                     "kernel": "k_res_steps<14, 2>", "primes": 2, "mul32_alg_per_squaring": mul32_alg(n, 2),
+                     "valu_issue_frac": valu_issue,
+                     "note": "achieved prices the two-prime work the kernel does (6 n (log2 n + 3) mul32 per "
+                             "squaring); valu_issue_frac = counted VALU wave-instructions per step x 4 clocks "
+                             "/ 1024 SIMDs / 2.4 GHz over the measured step time; traffic = HBM bytes per "
+                             "launch from FETCH_SIZE/WRITE_SIZE (state and tables read once per launch)"},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ct)

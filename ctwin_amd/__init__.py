@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libctwin_amd.so")
 
 KIND_AUTO, KIND_RESIDENT, KIND_PLAIN = 0, 1, 2
-TUNE_TIMINGS = 7                     # CT_TUNE_TIMINGS
+TUNE_TIMINGS = 10                    # CT_TUNE_TIMINGS
 
 STATUS = {0: "CT_OK", -1: "CT_ERR_ARG", -2: "CT_ERR_HIP", -3: "CT_ERR_NOMEM", -4: "CT_ERR_UNSUPPORTED",
           -5: "CT_ERR_GETINT", -6: "CT_ERR_COMPUTE", -7: "CT_ERR_GERBICZ"}
@@ -36,7 +36,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
            "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split",
            "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free",
-           "ct_sieve_factors_noalloc"]
+           "ct_sieve_factors_noalloc", "ct_engine_set_row_form", "ct_set_graphs_default"]
 
 INTERRUPTED = 1
 
@@ -110,8 +110,10 @@ def lib():
             "ct_context_save": [vp, c.c_char_p, i32, i32, c.c_double],
             "ct_context_load": [vp, c.c_char_p, c.POINTER(i32), c.POINTER(i32), c.POINTER(c.c_double)],
             "ct_cyclo_check_file": [i32, u32, c.c_char_p, c.c_char_p, u32, c.c_double, u64, c.POINTER(CycloStats)],
-            "ct_engine_tuning": [vp, c.POINTER(u32), c.POINTER(i32), c.POINTER(i32),
+            "ct_engine_tuning": [vp, c.POINTER(u32), c.POINTER(i32), c.POINTER(i32), c.POINTER(i32),
                                  np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
+            "ct_engine_set_row_form": [vp, i32],
+            "ct_set_graphs_default": [i32],
             "ct_engine_set_column_split": [vp, i32],
             "ct_engine_set_fused": [vp, i32],
             "ct_prp_begin": [vp, u32p, c.POINTER(vp), c.POINTER(i32), c.POINTER(i32)],
@@ -235,11 +237,19 @@ class Transform:
     def tuning(self):
         """(column tile, split of column pass A, fused column pass, ms per
         squaring for {resident, multipass W=16, W=32, W=16 split, W=32 split,
-        W=16 fused, W=32 fused}; -1 = not timed)."""
-        w, sp, fu = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int()
+        W=16 fused, W=32 fused, W=16 fused split, W=32 fused split, the
+        fastest with the sequential row form}; -1 = not timed, sequential row
+        form).  The row form is last so that the first four entries keep their
+        meaning."""
+        w, sp, fu, rs = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = np.zeros(TUNE_TIMINGS, np.float32)
-        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ctypes.byref(fu), ms))
-        return int(w.value), bool(sp.value), bool(fu.value), [float(x) for x in ms]
+        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ctypes.byref(fu), ctypes.byref(rs), ms))
+        return int(w.value), bool(sp.value), bool(fu.value), [float(x) for x in ms], bool(rs.value)
+
+    def set_row_form(self, seq):
+        """Multipass engine: row pass per (row, prime) workgroup (0) or one
+        workgroup per row transforming its primes in sequence (1)."""
+        _check(lib().ct_engine_set_row_form(self.h, int(bool(seq))))
 
     def set_fused(self, on):
         """Multipass engine (N+, n >= 2^13): column pass C fused with the next
@@ -403,6 +413,11 @@ class Server:
         q = ctypes.c_uint32(); a = ctypes.c_uint32(); u = ctypes.c_uint64(); s = ctypes.c_uint64()
         _check(lib().ct_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(u), ctypes.byref(s)))
         return {"queued": q.value, "active": a.value, "hbm_used": u.value, "squarings": s.value}
+
+
+def set_graphs_default(on):
+    """Graph replay for engines created from now on, the autotuner's included."""
+    _check(lib().ct_set_graphs_default(int(bool(on))))
 
 
 def moduli():

@@ -17,6 +17,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/ctwin.h"
 #include "ct_engine_impl.h"
 #include "ct_prp.h"
@@ -91,6 +93,8 @@ int ct_cyclo_check_file(int device, uint32_t log2_n, const char *cand_file, cons
 	else { rk = ri = 0; restored = 0; }
 	const double t0 = now_s() - restored;
 	double last_save = now_s();
+	constexpr uint64_t kAhead = 8;      // Gerbicz-Li segments in flight at most
+	uint64_t segments = 0;
 	const uint32_t n = 1u << log2_n;
 
 	for (size_t k = (size_t)rk; k < batches.size(); ++k)
@@ -129,6 +133,12 @@ int ct_cyclo_check_file(int device, uint32_t log2_n, const char *cand_file, cons
 			if ((rc = ct_engine_run(e, i, lo, B))) return rc;
 			st.squarings += (uint64_t)(i - lo + 1);
 			i = lo - 1;
+			// bounded run-ahead: the host checks the quit flag and the
+			// checkpoint clock as it enqueues segments, so it waits for the
+			// device every kAhead segments (the reference forces a queue finish
+			// every 1024 launches, ocl.h:541-545)
+			if (++segments % kAhead == 0 && hipStreamSynchronize(e->st) != hipSuccess)
+				return fail(CT_ERR_HIP, "engine stream failed");
 		}
 		std::vector<int> prm(lanes), glok(lanes);
 		std::vector<uint64_t> r64(lanes);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -44,6 +45,16 @@ int ct::hip_fail(hipError_t e, const char *where)
 // ---------------------------------------------------------------------------
 // engine lifetime
 
+// Graph replay for engines created from now on (the autotuner's timing
+// engines included); ct_engine_set_graphs changes one engine.
+static std::atomic<int> g_graphs_default{1};
+
+int ct_set_graphs_default(int on)
+{
+	g_graphs_default.store(on != 0 ? 1 : 0);
+	return CT_OK;
+}
+
 static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, uint32_t col_w, bool split, int sign,
                        ct_engine **out)
 {
@@ -62,6 +73,7 @@ static int create_impl(int device, uint32_t log2_n, uint32_t lanes, int kind, ui
 	e->device = device; e->ln = log2_n; e->n = 1u << log2_n; e->lanes = lanes; e->csize = 16; e->kind = kind;
 	e->sign = (sign < 0) ? -1 : 1; e->tln = tln; e->tn = 1u << tln;
 	e->col_w = col_w; e->split_a = split;
+	e->use_graphs = g_graphs_default.load() != 0;
 	e->gc = make_garner();
 	e->b.assign(lanes, 0);
 	const size_t reg_words = (size_t)lanes * e->n;
@@ -108,7 +120,7 @@ namespace {
 struct tune_key { int device; uint32_t ln, lanes; int kind, sign, np;
 	bool operator<(const tune_key &o) const
 	{ return std::tie(device, ln, lanes, kind, sign, np) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign, o.np); } };
-struct tune_val { int kind; uint32_t col_w; bool split, fused; float ms[CT_TUNE_TIMINGS]; };
+struct tune_val { int kind; uint32_t col_w; bool split, fused, row_seq; float ms[CT_TUNE_TIMINGS]; };
 std::mutex g_tune_mu;
 std::map<tune_key, tune_val> g_tune;
 
@@ -148,7 +160,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, int
 		auto it = g_tune.find(key);
 		if (it != g_tune.end()) { *out = it->second; return CT_OK; }
 	}
-	tune_val v{CT_KIND_AUTO, 0, false, false, {-1, -1, -1, -1, -1, -1, -1}};
+	tune_val v{CT_KIND_AUTO, 0, false, false, false, {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
 	float best = 1e30f;
 	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && tln <= 14);
 	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && tln >= (sign < 0 ? 11u : 10u));
@@ -179,24 +191,37 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, int
 				if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = (sp != 0); v.fused = false; }
 			}
 		if (sign > 0 && tln >= 13)
-			for (int t = 0; t < 2; ++t)
-			{
-				if (!plain_tile_ok((int)tln, tiles[t])) continue;
-				e->col_w = tiles[t]; e->split_a = false; e->use_fused = true;
-				float ms = time_squarings(e, np);
-				// a failed exchange surfaces at the next read of register 0
-				if (ms >= 0 && plain_materialize(e) != CT_OK) ms = -1;
-				e->use_fused = false;
-				if (ms < 0) continue;              // not a candidate
-				v.ms[5 + t] = ms;
-				if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = false; v.fused = true; }
-			}
+			for (int sp = 0; sp < (np == 2 ? 2 : 1); ++sp)       // sp: primes in separate workgroup halves
+				for (int t = 0; t < 2; ++t)
+				{
+					if (!plain_tile_ok((int)tln, tiles[t])) continue;
+					e->col_w = tiles[t]; e->split_a = (sp != 0); e->use_fused = true;
+					float ms = time_squarings(e, np);
+					// a failed exchange surfaces at the next read of register 0
+					if (ms >= 0 && plain_materialize(e) != CT_OK) ms = -1;
+					e->use_fused = false;
+					if (ms < 0) continue;              // not a candidate
+					v.ms[5 + t + 2 * sp] = ms;
+					if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = tiles[t]; v.split = (sp != 0); v.fused = true; }
+				}
 		if (!plain_tile_ok((int)tln, 16) && !plain_tile_ok((int)tln, 32))
 		{   // small n: one tile only
 			const float ms = time_squarings(e, np);
 			if (ms < 0) { ct_engine_destroy(e); return fail(CT_ERR_HIP, "autotuner: timing the multipass engine failed"); }
 			v.ms[1] = ms;
 			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = plain_default_tile((int)tln); }
+		}
+		if (v.kind == CT_KIND_PLAIN)
+		{   // the best multipass configuration with the row pass's primes in sequence
+			e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->row_seq = true; e->fused_w = 0;
+			float ms = time_squarings(e, np);
+			if (ms >= 0 && v.fused && plain_materialize(e) != CT_OK) ms = -1;
+			e->row_seq = false; e->use_fused = false;
+			if (ms >= 0)
+			{
+				v.ms[9] = ms;
+				if (ms < best) { best = ms; v.row_seq = true; }
+			}
 		}
 		ct_engine_destroy(e);
 	}
@@ -232,6 +257,7 @@ int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t cs
 	{
 		std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
 		(*out)->use_fused = v.fused;
+		(*out)->row_seq = v.row_seq;
 		(*out)->tuned = true;
 		(*out)->tuned_np = 3;
 	}
@@ -263,6 +289,14 @@ int ct_engine_set_fused(ct_engine *e, int on)
 	return CT_OK;
 }
 
+int ct_engine_set_row_form(ct_engine *e, int seq)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (e->kind != CT_KIND_PLAIN) return fail(CT_ERR_UNSUPPORTED, "the row pass belongs to the multipass engine");
+	e->row_seq = (seq != 0);
+	return CT_OK;
+}
+
 int ct_engine_set_column_split(ct_engine *e, int on)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
@@ -271,12 +305,13 @@ int ct_engine_set_column_split(ct_engine *e, int on)
 	return CT_OK;
 }
 
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, float *ms)
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, int *row_seq, float *ms)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
 	if (col_w) *col_w = (e->kind == CT_KIND_PLAIN) ? e->col_w : 0;
 	if (split) *split = (e->kind == CT_KIND_PLAIN && e->split_a) ? 1 : 0;
 	if (fused) *fused = (e->kind == CT_KIND_PLAIN && e->use_fused) ? 1 : 0;
+	if (row_seq) *row_seq = (e->kind == CT_KIND_PLAIN && e->row_seq) ? 1 : 0;
 	if (ms) for (int i = 0; i < CT_TUNE_TIMINGS; ++i) ms[i] = e->tune_ms[i];
 	return CT_OK;
 }
@@ -346,7 +381,7 @@ static int retune_for_primes(ct_engine *e)
 		(void)hipGetLastError();      // a failed allocation must not surface at the next launch check
 		return CT_OK;
 	}
-	e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->fused_w = 0;
+	e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->row_seq = v.row_seq; e->fused_w = 0;
 	for (int i = 1; i < CT_TUNE_TIMINGS; ++i) e->tune_ms[i] = v.ms[i];
 	return CT_OK;
 }

@@ -45,6 +45,7 @@ struct plain_args
 	int step;                   // squaring index within this launch sequence: bit = top - (iter + step)
 	int base_m1;                // PRP base - 1 (a set bit multiplies by the base)
 	int split_a;                // pass A: one workgroup per column group and prime
+	int row_seq;                // pass B: one workgroup per row, its primes in sequence
 	uint32_t lanes;
 	garner_consts gc;
 };
@@ -290,6 +291,57 @@ __global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
 	if (P == 0) rowB_prime<LN, RL, 0, MUL>(A, lds, r, lane, t);
 	else if (P == 1) rowB_prime<LN, RL, 1, MUL>(A, lds, r, lane, t);
 	else rowB_prime<LN, RL, 2, MUL>(A, lds, r, lane, t);
+}
+
+// B with the primes of a row in one workgroup, one after the other: the next
+// prime's row is requested before the current one is transformed (its loads
+// are in flight during the transform), and the current one is stored before
+// the next is transformed.  Half the workgroups of k_rowB, each with half of
+// its global-memory phases exposed.
+template <int LN, int RL, int P, int NP>
+__device__ __forceinline__ void rowS_prime(plain_args &A, uint32_t *lds, uint32_t r, uint32_t lane, uint32_t t,
+                                           uint32_t (&u)[E])
+{
+	constexpr int CL = LN - RL;
+	constexpr uint32_t n = 1u << LN, C = 1u << CL, T = C / 16, R = 1u << RL;
+	const size_t plane = (size_t)A.lanes * n;
+	uint32_t nx[E];
+	if constexpr (P + 1 < NP)
+	{
+		const uint32_t *nrow = A.res + (P + 1) * plane + (size_t)lane * n + (size_t)r * C;
+#pragma unroll
+		for (int q = 0; q < E; ++q) nx[q] = nrow[t + q * T];
+	}
+	const tw_entry *twf = A.tw + (size_t)(P * 2) * n, *twi = A.tw + (size_t)(P * 2 + 1) * n;
+	uint32_t dummy[E];
+	ftw<CL> F;
+	fwd_all<CL, P, false, true>(u, lds, twf, twi, R + r, t, F);
+	final_inv<CL, P, false, false, true>(u, dummy, lds, twf, twi, R + r, t, F);
+	uint32_t *row = A.res + P * plane + (size_t)lane * n + (size_t)r * C;
+#pragma unroll
+	for (int q = 0; q < E; ++q) row[t + q * T] = u[q];
+	if constexpr (P + 1 < NP)
+	{
+		__syncthreads();                 // every wave is done with this prime's LDS transposes
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = nx[q];
+		rowS_prime<LN, RL, P + 1, NP>(A, lds, r, lane, t, u);
+	}
+}
+
+template <int LN, int RL, int NP>
+__global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowS(plain_args A)
+{
+	constexpr int CL = LN - RL;
+	constexpr uint32_t n = 1u << LN, C = 1u << CL, T = C / 16;
+	__shared__ uint32_t lds[geo<CL>::lds_words];
+	const uint32_t r = blockIdx.x, lane = blockIdx.y, t = threadIdx.x;
+	if (A.lane_i && A.lane_i[lane] < 0) return;
+	const uint32_t *row0 = A.res + (size_t)lane * n + (size_t)r * C;
+	uint32_t u[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q) u[q] = row0[t + q * T];
+	rowS_prime<LN, RL, 0, NP>(A, lds, r, lane, t, u);
 }
 
 // C: inverse column levels, CRT, normalize1.  MINUS: the 2n coefficients of
@@ -598,6 +650,153 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 	                                      (size_t)A.lanes * n);
 }
 
+// k_colCA in two-prime mode with the primes in separate halves of a
+// workgroup twice as large (thread t: prime t / TH, column role t % TH): each
+// half runs its prime's inverse column levels, the first half the CRT,
+// normalize1, the carry exchange and normalize2 (the CRT reads both primes'
+// planes in LDS), then each half its prime's forward column levels from the
+// digits in LDS.  Twice the waves per column group, half the registers per
+// thread.  The exchange and its ordering are those of k_colCA.
+template <int LN, int RL, int W>
+__global__ __launch_bounds__((2 * colgeo<LN, RL, W>::threads)) void k_colCA2(plain_args A, int32_t *flags, int32_t *ctl,
+                                                                            int64_t *xc, int use_ticket)
+{
+	using G = colgeo<LN, RL, W>;
+	constexpr int NP = 2;
+	constexpr uint32_t n = 1u << LN, C = n >> RL, GW = C / G::W, hpr = G::W / 16, R = G::R, TH = G::threads;
+	static_assert(G::passes == 2 && TH % 64 == 0, "256-row columns; each half made of whole wavefronts");
+	__shared__ uint32_t S[NP][G::W * G::RP];
+	__shared__ uint32_t X[G::W * G::RP];
+	__shared__ int64_t fl[TH];
+	__shared__ uint32_t ticket;
+	const uint32_t tf = threadIdx.x;
+	const uint32_t P = tf / TH, t = tf % TH;             // P is uniform over a wavefront
+	uint32_t grp = blockIdx.x;
+	if (use_ticket)
+	{
+		if (tf == 0)
+		{
+			const uint32_t total = GW * A.lanes;
+			const uint32_t tk = (uint32_t)__hip_atomic_fetch_add(&ctl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (tk == total - 1) __hip_atomic_store(&ctl[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			ticket = tk;
+		}
+		__syncthreads();
+		grp = ticket;
+	}
+	const int32_t epoch = (int32_t)((uint32_t)__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
+	                                (uint32_t)A.step + 1u);
+	const uint32_t lane = grp / GW, j = grp % GW, c0 = j * G::W;
+	if (A.lane_i && A.lane_i[lane] < 0) return;
+	const uint32_t cw = t % G::W, tt = t / G::W;
+	const size_t plane = (size_t)A.lanes * n;
+	// 1. inverse column levels and the merge of prime P, into S[P] (layout 0)
+	{
+		uint32_t u[E];
+		const uint32_t *in = A.res + P * plane + (size_t)lane * n + c0 + cw;
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = in[(size_t)elem<RL, 1>(tt, q) * C];
+		const tw_entry *twi = A.tw + (size_t)(P * 2 + 1) * n;
+		if (P == 0) col_inv_hi<RL, 0>(u, twi, tt);
+		else col_inv_hi<RL, 1>(u, twi, tt);
+		uint32_t *sw = S[P] + cw * G::RP + lbase<RL, 1>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) sw[loff<RL, 1>(q)] = u[q];
+		__syncthreads();
+		uint32_t *sr = S[P] + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = sr[loff<RL, 0>(q)];
+		if (P == 0) col_inv_lo<RL, 0>(u, twi, tt);
+		else col_inv_lo<RL, 1>(u, twi, tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) sr[loff<RL, 0>(q)] = u[q];        // the words this thread read
+	}
+	__syncthreads();
+	// 2. first half: CRT and normalize1 of chunk (r, h), publish the row's last carry
+	const uint32_t r = t / hpr, h = t % hpr;
+	int32_t dig[E];
+	int64_t *xl = xc + (size_t)lane * GW * R;
+	int32_t *fg = flags + (size_t)lane * GW;
+	if (P == 0)
+	{
+		const uint32_t rr = r + (r >> 5);
+		const int bi = (A.lane_i ? A.lane_i[lane] : A.run[0]) - (A.run[1] + A.step);
+		bool dup = false;
+		if (A.bits) dup = (A.bits[(size_t)lane * A.bit_words + (bi >> 5)] >> (bi & 31)) & 1u;
+		else if (A.mask) dup = (A.mask[lane >> 5] >> (lane & 31)) & 1u;
+		int64_t l[E];
+#pragma unroll
+		for (int q = 0; q < E; ++q)
+		{
+			const uint32_t a = (16 * h + q) * G::RP + rr;
+			l[q] = garner2(S[0][a], S[1][a], A.gc);
+		}
+		const int64_t f = carry_chunk64(l, dig, dup, A.base_m1, A.dv[lane]);
+		fl[t] = f;
+		if (h == hpr - 1) __hip_atomic_store(&xl[(size_t)j * R + r], f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__builtin_amdgcn_s_waitcnt(0);
+	}
+	__syncthreads();
+	const uint32_t jp = (j + GW - 1) % GW;
+	if (tf == 0)
+	{
+		__hip_atomic_store(&fg[j], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint32_t spins = 0;
+		while (__hip_atomic_load(&fg[jp], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+		{
+			if (++spins > (1u << 22)) { __hip_atomic_store(&ctl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+			__builtin_amdgcn_s_sleep(2);
+		}
+	}
+	__syncthreads();
+	// normalize2 and the digits into D = S[0] (every read of S came before the barriers above)
+	int32_t *D = reinterpret_cast<int32_t *>(S[0]);
+	if (P == 0)
+	{
+		int64_t c;
+		if (h > 0) c = fl[t - 1];
+		else if (j > 0) c = __hip_atomic_load(&xl[(size_t)jp * R + r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		else
+		{
+			const int64_t top = __hip_atomic_load(&xl[(size_t)(GW - 1) * R + (R - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (r == 0) c = -top;
+			else
+			{
+				c = __hip_atomic_load(&xl[(size_t)(GW - 1) * R + (r - 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				if (r == R / 2) c += top;
+			}
+		}
+		carry_in(dig, c, A.dv[lane]);
+#pragma unroll
+		for (int q = 0; q < E; ++q) D[(16 * h + q) * G::RP + r + (r >> 5)] = dig[q];
+	}
+	__syncthreads();
+	// 3. forward column levels of prime P from the digits; prime 0 transposes
+	// through X, prime 1 through S[1] (free since the CRT)
+	{
+		uint32_t u[E];
+		const int32_t *dr = D + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = from_signed_np<NP>(dr[loff<RL, 0>(q)], kP[P]);
+		const tw_entry *twf = A.tw + (size_t)(P * 2) * n;
+		if (P == 0) col_fwd_lo<RL, 0>(u, twf, tt);
+		else col_fwd_lo<RL, 1>(u, twf, tt);
+		uint32_t *buf = (P == 0) ? X : S[1];
+		uint32_t *bw = buf + cw * G::RP + lbase<RL, 0>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) bw[loff<RL, 0>(q)] = u[q];
+		__syncthreads();
+		const uint32_t *br = buf + cw * G::RP + lbase<RL, 1>(tt);
+#pragma unroll
+		for (int q = 0; q < E; ++q) u[q] = br[loff<RL, 1>(q)];
+		if (P == 0) col_fwd_hi<RL, 0>(u, twf, tt);
+		else col_fwd_hi<RL, 1>(u, twf, tt);
+		uint32_t *out = A.res + P * plane + (size_t)lane * n + c0 + cw;
+#pragma unroll
+		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, 1>(tt, q) * C] = u[q];
+	}
+}
+
 // normalize2 alone: materialises the pending carries into the digits
 // n = digits per lane, rows = rows holding digits, minus: N- wrap
 __global__ void k_carry(int32_t *x, const int64_t *carry, const divisor *dvs, uint32_t n, uint32_t lanes, uint32_t rows,
@@ -653,7 +852,11 @@ static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st,
 		if (A.split_a) hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, true>), dim3(C / G::W, lanes, NP), dim3(G::threads), 0, st, A, 0);
 		else hipLaunchKernelGGL((k_colA<LN, RL, NP, W, MINUS, false>), gcol, dim3(G::threads), 0, st, A, 0);
 	}
-	if (parts & 2) hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
+	if (parts & 2)
+	{
+		if (A.row_seq) hipLaunchKernelGGL((k_rowS<LN, RL, NP>), dim3(R, lanes), dim3(C / 16), 0, st, A);
+		else hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
+	}
 	if (parts & 4) hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
 	return hipGetLastError();
 }
@@ -668,20 +871,26 @@ static hipError_t fused_ln(const plain_args &A, uint32_t lanes, hipStream_t st, 
 {
 	using G = colgeo<LN, 8, W>;
 	constexpr uint32_t C = 1u << (LN - 8);
+	const bool halves = (NP == 2) && A.split_a;           // the primes in separate halves (k_colCA2)
+	const void *fn = halves ? reinterpret_cast<const void *>(&k_colCA2<LN, 8, W>)
+	                        : reinterpret_cast<const void *>(&k_colCA<LN, 8, NP, W>);
+	const uint32_t threads = halves ? 2 * G::threads : G::threads;
 	if (fits)
 	{
 		int dev = 0, cus = 0, per = 0;
 		hipError_t h = hipGetDevice(&dev);
 		if (h == hipSuccess) h = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-		if (h == hipSuccess)
-			h = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void *>(&k_colCA<LN, 8, NP, W>),
-			                                                 (int)G::threads, 0);
+		if (h == hipSuccess) h = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, (int)threads, 0);
 		if (h != hipSuccess) return h;
 		*fits = (uint64_t)per * (uint64_t)cus >= (uint64_t)(C / G::W) * lanes;
 		return hipSuccess;
 	}
-	hipLaunchKernelGGL((k_colCA<LN, 8, NP, W>), dim3((C / G::W) * lanes), dim3(G::threads), 0, st, A, F.flags, F.ctl, F.xc,
-	                   F.use_ticket);
+	if (halves)
+		hipLaunchKernelGGL((k_colCA2<LN, 8, W>), dim3((C / G::W) * lanes), dim3(threads), 0, st, A, F.flags, F.ctl, F.xc,
+		                   F.use_ticket);
+	else
+		hipLaunchKernelGGL((k_colCA<LN, 8, NP, W>), dim3((C / G::W) * lanes), dim3(threads), 0, st, A, F.flags, F.ctl, F.xc,
+		                   F.use_ticket);
 	return hipGetLastError();
 }
 
@@ -849,6 +1058,7 @@ static plain_args make_args(ct_engine *e)
 	A.res = e->d_res; A.res2 = e->d_res2; A.carry = e->d_carry;
 	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc; A.run = e->d_run; A.base_m1 = (int)e->base - 1;
 	A.split_a = e->split_a ? 1 : 0;
+	A.row_seq = e->row_seq ? 1 : 0;
 	return A;
 }
 
@@ -927,6 +1137,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, bool fused, hipGraphEx
 	const bool fc = e->pending;
 	for (const auto &g : e->graphs)
 		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.base == e->base && g.split == e->split_a &&
+		    g.row_seq == e->row_seq &&
 		    g.fused == fused && g.ticket == (fused && e->fused_ticket) && g.first_carry == fc && g.bits == A.bits &&
 		    g.mask == A.mask &&
 		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
@@ -949,8 +1160,8 @@ static int graph_for(ct_engine *e, plain_args A, int len, bool fused, hipGraphEx
 	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	hipGraphDestroy(graph);
 	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
-	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, fused, fused && e->fused_ticket, fc, A.bits, A.mask,
-	                     A.lane_i, A.bit_words, exec});
+	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, e->row_seq, fused, fused && e->fused_ticket, fc, A.bits,
+	                     A.mask, A.lane_i, A.bit_words, exec});
 	*out = exec;
 	return CT_OK;
 }
@@ -965,7 +1176,7 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 	HIPCHK(hipGetLastError());
 	// the fused form: N+, 256-row layout
 	const bool fused = e->use_fused && e->sign > 0 && e->rows_log2 == 8 && nsteps > 1;
-	if (fused && (e->fused_w != e->col_w || e->fused_np != e->np))
+	if (fused && (e->fused_w != e->col_w || e->fused_np != e->np || e->fused_split != e->split_a))
 	{   // a new kernel or grid shape: clear the flags, the words and the epoch
 		// base; tickets only when the grid does not fit the device at once
 		bool fits = false;
@@ -975,6 +1186,7 @@ int plain_steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi
 		HIPCHK(hipMemsetAsync(e->d_flags, 0, e->fused_bytes, e->st));
 		e->fused_w = e->col_w;
 		e->fused_np = e->np;
+		e->fused_split = e->split_a;
 	}
 	if (e->use_graphs && nsteps > 1)
 	{

@@ -9,9 +9,12 @@
 // b, exponent window and bit offset are written for that lane only; the other
 // lanes keep running), and a new class engine is created only while the sum
 // of the engines' device allocations stays within the HBM budget; idle classes
-// are released when a queued request needs the memory.  Each round launches
-// min(chunk, shortest remaining) squarings on every busy class, all streams
-// first, then waits, so the classes' kernels overlap on the device.
+// are released when a queued request needs the memory.  Every class runs on
+// its own cadence: a chunk of min(chunk / 2^(ln - 14) (at least 4), shortest
+// remaining) squarings is launched on the class stream with an event behind it, and the class is read
+// out, re-admitted and relaunched as soon as that event has completed, while
+// the chunks of the other classes are still running (their kernels overlap on
+// the device).  A small class is thus never held back by a large one.
 #include <hip/hip_runtime.h>
 #include <gmp.h>
 
@@ -21,6 +24,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "../../include/ctwin.h"
@@ -48,8 +52,12 @@ struct size_class
 	std::vector<lane_state> lanes;
 	std::vector<uint32_t> stage_bits;
 	std::vector<int32_t> stage_digits;
+	std::vector<divisor> stage_dv;
+	std::vector<int32_t> stage_lane_i;      // lane offsets of the chunk in flight (read by its upload)
 	uint64_t bytes = 0;
-	bool launched = false;
+	hipEvent_t ev = nullptr;                // recorded behind the chunk in flight
+	bool in_flight = false;
+	int k = 0;                              // squarings of the chunk in flight
 };
 
 struct result { uint64_t id, res64, hash; double latency; };
@@ -122,6 +130,8 @@ static void destroy_class(ct_server *s, uint32_t ln)
 	auto it = s->classes.find(ln);
 	if (it == s->classes.end()) return;
 	size_class &c = *it->second;
+	if (c.e && c.e->st) (void)hipStreamSynchronize(c.e->st);
+	if (c.ev) (void)hipEventDestroy(c.ev);
 	hipFree(c.d_lane_i);
 	ct_engine_destroy(c.e);
 	s->used -= c.bytes;
@@ -165,7 +175,7 @@ static int make_class(ct_server *s, uint32_t ln, size_class **out)
 	int rc = ct_engine_create(s->device, ln, s->slots, 0, CT_KIND_AUTO, &c->e);
 	if (rc) return rc;
 	const uint64_t bytes = class_bytes(ln, s->slots, words, c->e->kind == CT_KIND_PLAIN);
-	c->e->use_graphs = s->graphs;
+	c->e->use_graphs = c->e->use_graphs && s->graphs;
 	c->ln = ln; c->words = words; c->bytes = bytes;
 	c->lanes.resize(s->slots);
 	c->lane_i.assign(s->slots, -1);
@@ -177,8 +187,12 @@ static int make_class(ct_server *s, uint32_t ln, size_class **out)
 	if ((rc = ct_engine_set_exponents(c->e, zeros.data(), words))) { ct_engine_destroy(c->e); return rc; }
 	if (hipMalloc((void **)&c->d_lane_i, s->slots * sizeof(int32_t)) != hipSuccess)
 	{ ct_engine_destroy(c->e); return fail(CT_ERR_NOMEM, "hipMalloc lane offsets"); }
+	if (hipEventCreateWithFlags(&c->ev, hipEventDisableTiming) != hipSuccess)
+	{ hipFree(c->d_lane_i); ct_engine_destroy(c->e); return fail(CT_ERR_HIP, "event create"); }
 	c->stage_bits.assign((size_t)s->slots * words, 0);
 	c->stage_digits.assign((size_t)s->slots * c->e->n, 0);
+	c->stage_dv.assign(s->slots, divisor{});
+	c->stage_lane_i.assign(s->slots, -1);
 	s->used += bytes;
 	*out = c.get();
 	s->classes[ln] = std::move(c);
@@ -202,14 +216,105 @@ static int admit(size_class &c, uint32_t l, const request &r)
 	e->b[l] = r.b;
 	e->b_max = std::max<uint64_t>(e->b_max, r.b);
 	choose_primes(e);
-	const divisor dv = make_divisor(r.b);
-	hipMemcpyAsync(e->d_dv + l, &dv, sizeof(divisor), hipMemcpyHostToDevice, e->st);
-	hipMemcpyAsync(e->d_bits + (size_t)l * c.words, bits, c.words * sizeof(uint32_t), hipMemcpyHostToDevice, e->st);
-	hipMemcpyAsync(e->reg(0) + (size_t)l * n, dig, n * sizeof(int32_t), hipMemcpyHostToDevice, e->st);
-	if (e->kind == CT_KIND_PLAIN) hipMemsetAsync(e->d_carry + (size_t)l * (n / 16), 0, (n / 16) * sizeof(int64_t), e->st);
-	hipStreamSynchronize(e->st);      // divisor staging lives on this stack frame
+	c.stage_dv[l] = make_divisor(r.b);
+	// stream-ordered uploads from per-class staging that is not touched again
+	// before the class's next chunk has completed
+	hipError_t h = hipMemcpyAsync(e->d_dv + l, &c.stage_dv[l], sizeof(divisor), hipMemcpyHostToDevice, e->st);
+	if (h == hipSuccess)
+		h = hipMemcpyAsync(e->d_bits + (size_t)l * c.words, bits, c.words * sizeof(uint32_t), hipMemcpyHostToDevice, e->st);
+	if (h == hipSuccess) h = hipMemcpyAsync(e->reg(0) + (size_t)l * n, dig, n * sizeof(int32_t), hipMemcpyHostToDevice, e->st);
+	if (h == hipSuccess && e->kind == CT_KIND_PLAIN)
+		h = hipMemsetAsync(e->d_carry + (size_t)l * (n / 16), 0, (n / 16) * sizeof(int64_t), e->st);
+	if (h != hipSuccess) return hip_fail(h, "admission upload");
 	c.lanes[l].busy = true; c.lanes[l].req = r; c.lanes[l].remaining = (int)r.steps;
 	c.lane_i[l] = (int32_t)r.steps - 1;
+	return CT_OK;
+}
+
+// The chunk in flight of class c has completed: advance its lanes and read
+// out the finished ones.
+static int complete_class(ct_server *s, size_class &c)
+{
+	c.in_flight = false;
+	ct_engine *e = c.e;
+	bool finished = false;
+	for (uint32_t l = 0; l < c.lanes.size(); ++l)
+		if (c.lanes[l].busy)
+		{
+			c.lanes[l].remaining -= c.k; c.lane_i[l] -= c.k; s->squarings += (uint64_t)c.k;
+			finished |= (c.lanes[l].remaining == 0);
+		}
+	if (!finished) return CT_OK;
+	int rc;
+	if (e->kind == CT_KIND_PLAIN && (rc = plain_materialize(e))) return rc;
+	if (hipStreamSynchronize(e->st) != hipSuccess) return fail(CT_ERR_HIP, "class stream failed");
+	const uint32_t n = e->n;
+	std::vector<int32_t> raw(n);
+	std::vector<uint32_t> canon(n);
+	for (uint32_t l = 0; l < c.lanes.size(); ++l)
+	{
+		lane_state &ls = c.lanes[l];
+		if (!(ls.busy && ls.remaining == 0)) continue;
+		if (hipMemcpy(raw.data(), e->reg(0) + (size_t)l * n, n * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+			return fail(CT_ERR_HIP, "read-out");
+		if (canonical(raw.data(), n, ls.req.b, 1, canon.data())) return fail(CT_ERR_GETINT, "getInt failed");
+		uint64_t r64 = canon[0], bi = ls.req.b, h = 1469598103934665603ull;
+		for (uint32_t j = 1; j < n; ++j) { r64 += (uint64_t)canon[j] * bi; bi *= ls.req.b; }
+		for (uint32_t j = 0; j < n; ++j) { h ^= canon[j]; h *= 1099511628211ull; }
+		s->done.push_back(result{ls.req.id, r64, h, now_s() - ls.req.t_submit});
+		ls.busy = false;
+		c.lane_i[l] = -1;
+	}
+	return CT_OK;
+}
+
+// Classes whose chunk has completed; with `wait`, until at least one has
+// (or none is in flight).  Returns the number completed, or a negative status.
+static int poll_classes(ct_server *s, bool wait)
+{
+	int done = 0;
+	for (;;)
+	{
+		bool any = false;
+		for (auto &kv : s->classes)
+		{
+			size_class &c = *kv.second;
+			if (!c.in_flight) continue;
+			any = true;
+			const hipError_t q = hipEventQuery(c.ev);
+			if (q == hipErrorNotReady) continue;
+			if (q != hipSuccess) return hip_fail(q, "class chunk");
+			const int rc = complete_class(s, c);
+			if (rc) return rc;
+			++done;
+		}
+		if (done || !wait || !any) return done;
+		std::this_thread::sleep_for(std::chrono::microseconds(20));
+	}
+}
+
+// Launches the next chunk of an idle class that has busy lanes.
+static int launch_class(ct_server *s, size_class &c)
+{
+	if (c.in_flight) return CT_OK;
+	// chunks sized by cost: a class of 2^ln digits launches chunk / 2^(ln - 14)
+	// squarings (at least 4), so that a large class's launch sequences stay
+	// short and the kernels of small classes that share its hardware queue
+	// are not held behind them
+	int k = std::max(4, (int)(s->chunk >> (c.ln > 14 ? c.ln - 14 : 0)));
+	bool any = false;
+	for (auto &ls : c.lanes) if (ls.busy) { any = true; k = std::min(k, ls.remaining); }
+	if (!any) return CT_OK;
+	ct_engine *e = c.e;
+	c.stage_lane_i = c.lane_i;            // read by the upload below; untouched until the chunk completes
+	hipError_t h = hipMemcpyAsync(c.d_lane_i, c.stage_lane_i.data(), c.stage_lane_i.size() * sizeof(int32_t),
+	                              hipMemcpyHostToDevice, e->st);
+	if (h != hipSuccess) return hip_fail(h, "lane offsets upload");
+	const int rc = steps(e, e->d_bits, c.words, 0, k, nullptr, c.d_lane_i);
+	if (rc) return rc;
+	if ((h = hipEventRecord(c.ev, e->st)) != hipSuccess) return hip_fail(h, "event record");
+	c.in_flight = true;
+	c.k = k;
 	return CT_OK;
 }
 
@@ -218,8 +323,11 @@ int ct_server_step(ct_server *s)
 	if (!s) return fail(CT_ERR_ARG, "null server");
 	int rc = hipSetDevice(s->device) == hipSuccess ? CT_OK : fail(CT_ERR_HIP, "hipSetDevice");
 	if (rc) return rc;
-	// 1. admission: FIFO, a request may overtake one whose class is full or
-	//    whose class cannot be allocated within the budget
+	// 1. classes whose chunk has completed: read out, free their lanes
+	if ((rc = poll_classes(s, false)) < 0) return rc;
+	// 2. admission: FIFO into free lanes of classes that are not running a
+	//    chunk; a request may overtake one whose class is full, running, or
+	//    cannot be allocated within the budget
 	for (auto it = s->queue.begin(); it != s->queue.end();)
 	{
 		const request r = *it;
@@ -233,7 +341,7 @@ int ct_server_step(ct_server *s)
 				std::vector<uint32_t> idle;
 				for (auto &kv : s->classes)
 				{
-					bool any = false;
+					bool any = kv.second->in_flight;
 					for (auto &ls : kv.second->lanes) any |= ls.busy;
 					if (!any) idle.push_back(kv.first);
 				}
@@ -247,58 +355,18 @@ int ct_server_step(ct_server *s)
 			}
 			if (rc) return rc;
 		}
+		if (c->in_flight) { ++it; continue; }
 		uint32_t l = 0;
 		while (l < c->lanes.size() && c->lanes[l].busy) ++l;
 		if (l == c->lanes.size()) { ++it; continue; }
 		if ((rc = admit(*c, l, r))) return rc;
 		it = s->queue.erase(it);
 	}
-	// 2. launch a chunk on every busy class (all streams before any wait)
+	// 3. launch every idle class that has work (streams overlap)
 	for (auto &kv : s->classes)
-	{
-		size_class &c = *kv.second;
-		c.launched = false;
-		int k = (int)s->chunk;
-		bool any = false;
-		for (auto &ls : c.lanes) if (ls.busy) { any = true; k = std::min(k, ls.remaining); }
-		if (!any) continue;
-		ct_engine *e = c.e;
-		hipMemcpyAsync(c.d_lane_i, c.lane_i.data(), c.lane_i.size() * sizeof(int32_t), hipMemcpyHostToDevice, e->st);
-		if ((rc = steps(e, e->d_bits, c.words, 0, k, nullptr, c.d_lane_i))) return rc;
-		for (uint32_t l = 0; l < c.lanes.size(); ++l)
-			if (c.lanes[l].busy) { c.lanes[l].remaining -= k; c.lane_i[l] -= k; s->squarings += k; }
-		c.launched = true;
-	}
-	// 3. wait, read out finished lanes
-	for (auto &kv : s->classes)
-	{
-		size_class &c = *kv.second;
-		if (!c.launched) continue;
-		ct_engine *e = c.e;
-		if (hipStreamSynchronize(e->st) != hipSuccess) return fail(CT_ERR_HIP, "class stream failed");
-		bool finished = false;
-		for (auto &ls : c.lanes) finished |= (ls.busy && ls.remaining == 0);
-		if (!finished) continue;
-		if (e->kind == CT_KIND_PLAIN && (rc = plain_materialize(e))) return rc;
-		hipStreamSynchronize(e->st);
-		const uint32_t n = e->n;
-		std::vector<int32_t> raw(n);
-		std::vector<uint32_t> canon(n);
-		for (uint32_t l = 0; l < c.lanes.size(); ++l)
-		{
-			lane_state &ls = c.lanes[l];
-			if (!(ls.busy && ls.remaining == 0)) continue;
-			if (hipMemcpy(raw.data(), e->reg(0) + (size_t)l * n, n * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
-				return fail(CT_ERR_HIP, "read-out");
-			if (canonical(raw.data(), n, ls.req.b, 1, canon.data())) return fail(CT_ERR_GETINT, "getInt failed");
-			uint64_t r64 = canon[0], bi = ls.req.b, h = 1469598103934665603ull;
-			for (uint32_t j = 1; j < n; ++j) { r64 += (uint64_t)canon[j] * bi; bi *= ls.req.b; }
-			for (uint32_t j = 0; j < n; ++j) { h ^= canon[j]; h *= 1099511628211ull; }
-			s->done.push_back(result{ls.req.id, r64, h, now_s() - ls.req.t_submit});
-			ls.busy = false;
-			c.lane_i[l] = -1;
-		}
-	}
+		if ((rc = launch_class(s, *kv.second))) return rc;
+	// 4. wait until some class completes its chunk (not all of them)
+	if ((rc = poll_classes(s, true)) < 0) return rc;
 	return CT_OK;
 }
 

@@ -73,7 +73,9 @@ int ct_engine_set_column_tile(ct_engine *e, uint32_t w);
 
 /* Multipass engine: column pass A as one workgroup per column group (on = 0)
  * or one per column group and prime (on = 1: twice the workgroups, both load
- * the digits).  Results do not depend on it.  No counterpart in the reference. */
+ * the digits); with the fused column pass and two primes, on = 1 puts the
+ * primes in separate halves of a workgroup twice as large.  Results do not
+ * depend on it.  No counterpart in the reference. */
 int ct_engine_set_column_split(ct_engine *e, int on);
 
 /* Multipass engine (N+, n >= 2^13): fuse column pass C of each squaring with
@@ -87,13 +89,21 @@ int ct_engine_set_column_split(ct_engine *e, int on);
  * reference. */
 int ct_engine_set_fused(ct_engine *e, int on);
 
+/* Multipass engine: row pass B as one workgroup per row and prime (seq = 0)
+ * or one workgroup per row that transforms its primes one after the other,
+ * loading the next prime's row while it transforms the current one (seq = 1:
+ * half the workgroups).  Results do not depend on it.  No counterpart in the
+ * reference. */
+int ct_engine_set_row_form(ct_engine *e, int seq);
+
 /* What the autotuner chose and measured: the column tile (0 for resident
- * kernels), the split of column pass A, the fused column pass, and ms per
- * squaring (-1 where not timed or when the engine was not tuned) in
- * ms[CT_TUNE_TIMINGS]: {resident, multipass W=16, W=32, W=16 split,
- * W=32 split, W=16 fused, W=32 fused}.  Any output may be null. */
-#define CT_TUNE_TIMINGS 7
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, float *ms);
+ * kernels), the split of the column passes, the fused column pass, the row
+ * form, and ms per squaring (-1 where not timed or when the engine was not
+ * tuned) in ms[CT_TUNE_TIMINGS]: {resident, multipass W=16, W=32, W=16 split,
+ * W=32 split, W=16 fused, W=32 fused, W=16 fused split, W=32 fused split, the
+ * fastest of these with the sequential row form}.  Any output may be null. */
+#define CT_TUNE_TIMINGS 10
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, int *row_seq, float *ms);
 int ct_engine_destroy(ct_engine *e);
 
 /* transform::set_b (transform.h:472-486): b[lanes], 32 < b < 2^31. */
@@ -192,6 +202,11 @@ int ct_engine_primes(const ct_engine *e, int *np);
  * run a whole run in one launch. */
 int ct_engine_set_graphs(ct_engine *e, int on);
 
+/* ct_engine_set_graphs for every engine created after the call, the
+ * autotuner's timing engines included (default on).  Off keeps HIP graphs out
+ * of a process altogether, e.g. one traced by a profiler. */
+int ct_set_graphs_default(int on);
+
 /* Engine facts: n, lanes, kind actually used. */
 int ct_engine_info(const ct_engine *e, uint32_t *n, uint32_t *lanes, int *kind);
 
@@ -210,14 +225,18 @@ int ct_engine_moduli(uint32_t *p);
  * yields x = 2^(top `steps` bits of N - 1) mod N, N = b^n - b^(n/2) + 1.
  * Requests are admitted into free lanes of per-size engines as lanes free up;
  * a new size class is created only within hbm_budget bytes of device memory.
- * slots = lanes per size class; chunk = squarings per scheduling round;
+ * slots = lanes per size class; chunk = squarings per launch of a class of up
+ * to 2^14 digits (larger classes launch chunk / 2^(log2_n - 14), at least 4);
  * max_steps bounds `steps`.  10 <= log2_n <= 22. */
 typedef struct ct_server ct_server;
 int ct_server_create(int device, uint64_t hbm_budget, uint32_t slots, uint32_t chunk, uint32_t max_steps, ct_server **out);
 int ct_server_destroy(ct_server *s);
 int ct_server_submit(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps);
-/* One round: admit what fits, launch a chunk on every busy size class (their
- * streams overlap), wait, read out finished requests. */
+/* One step: read out the size classes whose chunk has completed, admit
+ * queued requests into free lanes of classes that are not running a chunk,
+ * launch the next chunk on every idle class with work (their streams
+ * overlap), then wait until at least one class's chunk completes -- not all
+ * of them, so a small class is never paced by a large one. */
 int ct_server_step(ct_server *s);
 /* Finished requests since the last poll: id, RES64 of the canonical residue,
  * a 64-bit FNV-1a hash of its digits, latency from submit (s). */

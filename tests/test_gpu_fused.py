@@ -21,14 +21,16 @@ def _exponents(bs, n):
     return [top << s for top, s in (O.top_bits_of_exponent(b, n, 512) for b in bs)]
 
 
-@pytest.mark.parametrize("ln,lanes,tile,graphs,mode", [(13, 1, 16, True, 1), (13, 1, 16, False, 1), (16, 2, 16, True, 1),
-                                                       (16, 3, 32, False, 1), (16, 3, 16, True, 2), (21, 1, 16, True, 1),
-                                                       (21, 1, 32, True, 1), (21, 1, 32, False, 1), (21, 1, 32, True, 2)])
-def test_fused_equals_three_pass(ln, lanes, tile, graphs, mode):
+@pytest.mark.parametrize("ln,lanes,tile,graphs,mode,seq,split", [
+    (13, 1, 16, True, 1, False, False), (13, 1, 16, False, 1, True, True), (16, 2, 16, True, 1, False, True),
+    (16, 3, 32, False, 1, True, False), (16, 3, 16, True, 2, False, True), (21, 1, 16, True, 1, False, False),
+    (21, 1, 32, True, 1, False, False), (21, 1, 32, False, 1, False, True), (21, 1, 32, True, 2, False, True),
+    (21, 1, 32, True, 1, True, True)])
+def test_fused_equals_three_pass(ln, lanes, tile, graphs, mode, seq, split):
     n = 1 << ln
     rng = random.Random(1500 + ln + tile)
     bs = [rng.randrange(1 << 16, 1 << 18) for _ in range(lanes)]
-    print(f"\n[case] fused ln={ln} lanes={lanes} tile={tile} graphs={graphs} mode={mode} bs={bs}", flush=True)
+    print(f"\n[case] fused ln={ln} lanes={lanes} tile={tile} graphs={graphs} mode={mode} seq={seq} split={split} bs={bs}", flush=True)
     E = _exponents(bs, n)
     i0 = max(e.bit_length() for e in E) - 1
     engines = []
@@ -37,6 +39,8 @@ def test_fused_equals_three_pass(ln, lanes, tile, graphs, mode):
         t.set_column_tile(tile)
         t.set_fused(fused)
         t.set_graphs(graphs)
+        t.set_row_form(seq and fused)           # the reference engine keeps one workgroup per row and prime
+        t.set_column_split(split and fused)     # and one prime per thread
         t.set_b(bs); t.set_exponents(E); t.set(1); t.copy(1, 0)
         engines.append(t)
     i = i0

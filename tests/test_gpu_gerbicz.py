@@ -76,7 +76,7 @@ def test_gerbicz_detects_altered_register(ln, kind, fused):
 def test_quit_from_another_thread_then_resume(tmp_path):
     ln, lanes = 12, 8
     rng = random.Random(4242)
-    bs = [rng.randrange(1 << 17, 1 << 18) for _ in range(12)]
+    bs = [rng.randrange(1 << 17, 1 << 18) for _ in range(24)]      # three batches
     cand = tmp_path / "cand.txt"
     cand.write_text("".join(f"{b}\n" for b in bs))
     print(f"\n[case] quit/resume ln={ln} lanes={lanes} candidates={len(bs)}", flush=True)

@@ -28,7 +28,7 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     bs = [rng.randrange(1 << 17, 1 << 20) for _ in range(lanes)]
     print(f"\n[case] tuned ln={ln} lanes={lanes}", flush=True)
     tuned = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
-    w, split, fused, ms = tuned.tuning()
+    w, split, fused, ms, rs = tuned.tuning()
     assert len(ms) == ct.TUNE_TIMINGS
     timed = [m for m in ms if m > 0]
     assert timed, ms
@@ -37,29 +37,28 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     # the chosen configuration is the fastest one timed
     best = min(timed)
     if tuned.kind == ct.KIND_RESIDENT:
-        assert ms[0] == best and w == 0 and not split and not fused
+        assert ms[0] == best and w == 0 and not split and not fused and not rs
     else:
         assert w in (16, 32, 64)
         if fused:
-            assert not split
-            idx = 5 if w == 16 else 6
+            idx = (5 if w == 16 else 6) + (2 if split else 0)
         else:
             idx = (1 if w in (16, 64) else 2) + (2 if split else 0)
-        assert ms[idx] == best
+        assert ms[9 if rs else idx] == best
     ref = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=16)
     a = _run(tuned, bs, n, 40, 1)
     b = _run(ref, bs, n, 40, 1)
     assert np.array_equal(a, b)
     # set_b moved the engine to tuned.primes primes; a multipass engine is
     # tuned again for that count and runs the fastest configuration timed
-    w2, split2, fused2, ms2 = tuned.tuning()
+    w2, split2, fused2, ms2, rs2 = tuned.tuning()
     if tuned.kind == ct.KIND_PLAIN:
         timed2 = [m for m in ms2[1:] if m > 0]
-        idx2 = (5 if w2 == 16 else 6) if fused2 else (1 if w2 in (16, 64) else 2) + (2 if split2 else 0)
-        assert ms2[idx2] == min(timed2)
+        idx2 = ((5 if w2 == 16 else 6) if fused2 else (1 if w2 in (16, 64) else 2)) + (2 if split2 else 0)
+        assert ms2[9 if rs2 else idx2] == min(timed2)
     # second creation hits the cache: same choice, same timings
     again = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
-    assert again.kind == tuned.kind and again.tuning() == (w, split, fused, ms)
+    assert again.kind == tuned.kind and again.tuning() == (w, split, fused, ms, rs)
 
 
 @pytest.mark.parametrize("ln", [13, 16, 20])
@@ -71,10 +70,12 @@ def test_column_tiles_give_identical_digits(ln):
     out = []
     for w in (16, 32):
         for split in (False, True):
-            t = ct.Transform(ln, 1, ct.KIND_PLAIN)
-            t.set_column_tile(w)
-            t.set_column_split(split)
-            assert t.tuning()[:3] == (w, split, False)
-            out.append(_run(t, bs, n, 30, 2))
+            for seq in (False, True):
+                t = ct.Transform(ln, 1, ct.KIND_PLAIN)
+                t.set_column_tile(w)
+                t.set_column_split(split)
+                t.set_row_form(seq)
+                assert t.tuning()[:3] == (w, split, False) and t.tuning()[4] == seq
+                out.append(_run(t, bs, n, 30, 2))
     for o in out[1:]:
         assert np.array_equal(out[0], o)
