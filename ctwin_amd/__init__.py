@@ -36,7 +36,9 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
            "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split",
            "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free",
-           "ct_sieve_factors_noalloc", "ct_engine_set_row_form", "ct_set_graphs_default"]
+           "ct_sieve_factors_noalloc", "ct_engine_set_row_form", "ct_set_graphs_default",
+           "ct_fsieve_create", "ct_fsieve_blocks", "ct_fsieve_set_bmax", "ct_fsieve_run", "ct_fsieve_destroy",
+           "ct_fsieve_check", "ct_fsieve_quit"]
 
 INTERRUPTED = 1
 
@@ -44,6 +46,11 @@ INTERRUPTED = 1
 class CycloStats(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_uint64), ("prps", ctypes.c_uint64), ("squarings", ctypes.c_uint64),
                 ("elapsed", ctypes.c_double), ("resumed", ctypes.c_int)]
+
+
+class FsieveStats(ctypes.Structure):
+    _fields_ = [("blocks_done", ctypes.c_uint64), ("blocks_total", ctypes.c_uint64), ("factors", ctypes.c_uint64),
+                ("elapsed", ctypes.c_double), ("resumed", ctypes.c_int), ("interrupted", ctypes.c_int)]
 
 
 class CtError(RuntimeError):
@@ -114,6 +121,12 @@ def lib():
                                  np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
             "ct_engine_set_row_form": [vp, i32],
             "ct_set_graphs_default": [i32],
+            "ct_fsieve_create": [i32, i32, u64, u64, i32, c.POINTER(vp)],
+            "ct_fsieve_blocks": [vp, c.POINTER(u64), c.POINTER(u64)],
+            "ct_fsieve_set_bmax": [vp, u64],
+            "ct_fsieve_run": [vp, u64, u64, u64p, u64, c.POINTER(u64)],
+            "ct_fsieve_destroy": [vp],
+            "ct_fsieve_check": [i32, i32, u64, u64, i32, c.c_char_p, c.c_double, u64, c.POINTER(FsieveStats)],
             "ct_engine_set_column_split": [vp, i32],
             "ct_engine_set_fused": [vp, i32],
             "ct_prp_begin": [vp, u32p, c.POINTER(vp), c.POINTER(i32), c.POINTER(i32)],
@@ -128,6 +141,8 @@ def lib():
         L.ct_crc32.restype = u32
         L.ct_cyclo_quit.argtypes = []
         L.ct_cyclo_quit.restype = None
+        L.ct_fsieve_quit.argtypes = []
+        L.ct_fsieve_quit.restype = None
         L.ct_last_error.argtypes = []
         L.ct_last_error.restype = c.c_char_p
         _lib = L
@@ -455,6 +470,62 @@ def sieve_factors(m, p_min, p_max, mode, device=0, cap=1 << 20):
         _check(rc)
         out = pairs[:2 * cnt.value].reshape(-1, 2)
         return out[np.lexsort((out[:, 1], out[:, 0]))]
+
+
+class FactorSieve:
+    """Block cursor of the GPU factor emission (gfsieve's per-block driving):
+    blocks() = (i_min, i_max); run(i, nblocks) = pairs (p, b) of those blocks,
+    sorted."""
+
+    def __init__(self, m, p_min, p_max, mode, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().ct_fsieve_create(device, m, int(p_min), int(p_max), mode, ctypes.byref(h)))
+        self.h = h
+
+    def blocks(self):
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().ct_fsieve_blocks(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_bmax(self, bmax):
+        """0: the reference's bMax(p); otherwise pairs with b <= bmax (2^64 - 1: every root)."""
+        _check(lib().ct_fsieve_set_bmax(self.h, int(bmax)))
+
+    def run(self, i_begin, nblocks, cap=1 << 20):
+        while True:
+            pairs = np.zeros(2 * max(cap, 1), np.uint64)
+            cnt = ctypes.c_uint64()
+            rc = lib().ct_fsieve_run(self.h, int(i_begin), int(nblocks), pairs, cap, ctypes.byref(cnt))
+            if rc == -3 and cnt.value > cap:
+                cap = cnt.value
+                continue
+            _check(rc)
+            out = pairs[:2 * cnt.value].reshape(-1, 2)
+            return out[np.lexsort((out[:, 1], out[:, 0]))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ct_fsieve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def fsieve_check(m, p_min, p_max, mode, work_dir, record_s=120.0, max_blocks=0, device=0):
+    """gfsieve::check on the device: fp/fn<ext> factor files and the ctx<ext>
+    checkpoint in work_dir; returns (status, stats dict), status 0 = done,
+    INTERRUPTED = stopped with the checkpoint written."""
+    st = FsieveStats()
+    rc = lib().ct_fsieve_check(device, m, int(p_min), int(p_max), mode, os.fsencode(work_dir), float(record_s),
+                               int(max_blocks), ctypes.byref(st))
+    if rc not in (0, INTERRUPTED):
+        _check(rc)
+    return rc, {f: getattr(st, f) for f, _ in FsieveStats._fields_}
+
+
+def fsieve_quit():
+    lib().ct_fsieve_quit()
 
 
 def crc32(data, crc=0):

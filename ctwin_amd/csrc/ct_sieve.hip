@@ -410,70 +410,6 @@ __global__ void k_strike_small(const uint2 *small, const uint32_t *small_count, 
 	for (uint64_t t = s + threadIdx.x * p; t <= b_max; t += (uint64_t)blockDim.x * p) struck[t - b_min] = 1;
 }
 
-// ---------------------------------------------------------------------------
-// host fallback for the first p >= 2^32 that the reference's bound rule
-// includes (k_max is the first k with p(k) >= p_max, ctsieve.cpp:684, 734)
-
-static bool host_isprime(uint64_t p)
-{
-	if (p < 2) return false;
-	const uint32_t small[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
-	for (uint32_t s : small) { if (p == s) return true; if (p % s == 0) return false; }
-	m64h A(p);
-	return mr_prime<m64h, uint64_t>(A, p, small, 12);   // deterministic for p < 3.3e24
-}
-
-static void host_strike(uint8_t *struck, uint64_t x, uint64_t p, uint64_t b_min, uint64_t b_max)
-{
-	uint64_t s = x;
-	if (s < b_min) s += ((b_min - s + p - 1) / p) * p;
-	for (; s <= b_max; s += p) struck[s - b_min] = 1;
-}
-
-static void host_pos(uint64_t p, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck)
-{
-	const m64h A(p);
-	const uint64_t k = (p - 1) / (3ull << m);
-	uint64_t c = 0;
-	for (uint64_t a = 2;; ++a)
-	{
-		c = A.pow(a, k);
-		uint64_t c2 = c;
-		for (int i = 0; i < m - 1; ++i) c2 = A.mul(c2, c2);
-		if (A.mul(A.mul(c2, c2), c2) == p - 1 && c2 != p - 1) break;
-	}
-	const uint64_t c2 = A.mul(c, c), c4 = A.mul(c2, c2), c5 = A.mul(c4, c), g = A.mul(c4, c2);
-	uint64_t cur = 1;
-	for (uint64_t t = 0; t < (1ull << (m - 1)); ++t)
-	{
-		host_strike(struck, A.mul(c, cur), p, b_min, b_max);
-		host_strike(struck, A.mul(c5, cur), p, b_min, b_max);
-		cur = A.mul(cur, g);
-	}
-}
-
-static void host_neg(uint64_t p, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck)
-{
-	const m64h A(p);
-	uint64_t q = p - 1; int v = 0;
-	while ((q & 1) == 0) { q >>= 1; ++v; }
-	const uint64_t w = two_sylow_gen<m64h, uint64_t>(A, p, q);
-	const uint64_t s5 = root_2d<m64h, uint64_t>(A, p, q, v, w, 5, 1);
-	if (s5 == 0) return;
-	const uint64_t inv2 = (p + 1) / 2;
-	const int d = m - 1, e = (v < d) ? v : d;
-	uint64_t u = w;
-	for (int i = 0; i < v - e; ++i) u = A.mul(u, u);
-	for (int side = 0; side < 2; ++side)
-	{
-		const uint64_t y = A.mul(side ? A.add(1, s5) : A.sub(1, s5), inv2);
-		const uint64_t x = root_2d<m64h, uint64_t>(A, p, q, v, w, y, d);
-		if (x == 0) continue;
-		uint64_t r = x;
-		for (uint64_t j = 0; j < (1ull << e); ++j) { host_strike(struck, r, p, b_min, b_max); r = A.mul(r, u); }
-	}
-}
-
 }   // namespace ct
 
 using namespace ct;
@@ -494,13 +430,12 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 	const uint64_t kn_min = p_min_neg / 10;
 	uint64_t kn_max = p_max_neg / 10;
 	while (10 * kn_max + 1 < p_max_neg) ++kn_max;
-	// an empty k range (kp_max < kp_min, or kn_max == 0 when p_max_neg <= 1)
-	// processes nothing, as check_pos / check_neg do for it
-	if (mode >= 0 && kp_max >= kp_min && kp_max > 0 && 3 * (kp_max << m) + 1 >= P32 &&
-	    3 * ((kp_max - 1) << m) + 1 >= P32)
-		return fail(CT_ERR_UNSUPPORTED, "+ side: p beyond 2^32");
-	if (mode <= 0 && kn_max >= kn_min && kn_max > 0 && 10 * kn_max - 1 >= P32 && 10 * (kn_max - 1) + 1 >= P32)
-		return fail(CT_ERR_UNSUPPORTED, "- side: p beyond 2^32");
+	if ((mode >= 0 && p_max_pos >= (1ull << 62)) || (mode <= 0 && p_max_neg >= (1ull << 62)))
+		return fail(CT_ERR_UNSUPPORTED, "p bounds must lie below 2^62");
+	// primes from 2^32 up run through the 64-bit kernels: the k from which
+	// every p of the side is >= 2^32
+	const uint64_t kp64 = (P32 - 1 + (3ull << m) - 1) / (3ull << m);     // 3 k 2^m + 1 >= 2^32
+	const uint64_t kn64 = (P32 + 10) / 10;                                // 10 k - 1 >= 2^32
 
 	const size_t W = (size_t)(b_max - b_min + 1);
 	const uint64_t KB = 1ull << 24;                      // k values per block
@@ -594,22 +529,20 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 				}
 			}
 		}
+		// p >= 2^32 (up to the first p >= p_max, the reference's rule): the
+		// 64-bit prime and root kernels strike the same byte map
+		if (mode >= 0 && kp_max >= kp_min && kp_max >= kp64)
+		{
+			rc = strike_window64(st, m, +1, std::max(kp_min, kp64), kp_max, P32, 3 * (kp_max << m) + 1, b_min, b_max, d_struck);
+			if (rc) goto done;
+		}
+		if (mode <= 0 && kn_max >= kn_min && kn_max >= kn64)
+		{
+			rc = strike_window64(st, m, -1, std::max(kn_min, kn64), kn_max, P32, 10 * kn_max + 1, b_min, b_max, d_struck);
+			if (rc) goto done;
+		}
 		HIPCHK(hipStreamSynchronize(st));
 		HIPCHK(hipMemcpy(struck, d_struck, W, hipMemcpyDeviceToHost));
-		// boundary primes >= 2^32 included by the reference's rule
-		if (mode >= 0 && kp_max >= kp_min)
-		{
-			const uint64_t p = 3 * (kp_max << m) + 1;
-			if (p >= P32 && host_isprime(p)) host_pos(p, m, b_min, b_max, struck);
-		}
-		if (mode <= 0 && kn_max >= kn_min)
-		{
-			for (int s = 0; s < 2; ++s)
-			{
-				const uint64_t p = 10 * kn_max + 2 * s - 1;
-				if (p >= P32 && host_isprime(p)) host_neg(p, m, b_min, b_max, struck);
-			}
-		}
 	}
 done:
 	hipFree(d_struck); hipFree(d_list); hipFree(d_count); hipFree(d_scount); hipFree(d_small); hipFree(d_big); hipFree(d_sp);

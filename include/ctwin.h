@@ -250,11 +250,15 @@ int ct_server_set_graphs(ct_server *s, int on);
 
 /* Sieve::check_pos / check_neg (sieve/CPU/ctsieve.cpp:678-787) on the GPU:
  * strike every b in [b_min, b_max] with a prime factor p of
- * b^(2^m) - b^(2^(m-1)) + 1 (p = 3k 2^m + 1, p_min_pos <= p up to the first
- * p >= p_max_pos; mode >= 0) or of b^(2^m) - b^(2^(m-1)) - 1 (p = 10k +/- 1,
- * same bounds rule with p_min_neg / p_max_neg; mode <= 0).  struck[b - b_min]
- * is read and updated (1 = struck); p < 2^32.  The survivor list is the set of
- * unstruck b in ascending order (cand_*.txt of ctsieve.cpp:621-626). */
+ * b^(2^m) - b^(2^(m-1)) + 1 (p = 3k 2^m + 1 for k from (p_min_pos / 2^m) / 3
+ * up to the first p >= p_max_pos; mode >= 0) or of b^(2^m) - b^(2^(m-1)) - 1
+ * (p = 10k +/- 1 for k from p_min_neg / 10, same bounds rule with p_max_neg;
+ * mode <= 0).  struck[b - b_min] is read and updated (1 = struck); p bounds
+ * below 2^62 (the reference's ini bounds, ctsieve.cpp:797-801, and beyond),
+ * b_max - b_min < 2^32.  Primes below 2^32 run through 32-bit kernels, the
+ * others through the 64-bit ones of the factor emission.  The survivor list
+ * is the set of unstruck b in ascending order (cand_*.txt of
+ * ctsieve.cpp:621-626). */
 int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_max_pos,
                     uint64_t p_min_neg, uint64_t p_max_neg, int mode, uint8_t *struck);
 
@@ -287,6 +291,49 @@ int ct_sieve_factors(int device, int m, uint64_t p_min, uint64_t p_max, int mode
  * on a device shares that array: calls that overlap on one device race. */
 int ct_sieve_factors_noalloc(int device, int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs,
                              uint64_t cap, uint64_t *count, void *stream);
+
+/* Block cursor over the GPU factor emission, the per-block driving of
+ * gfsieve::check (sieve/GPU/src/ctsieve.h:366-409) and engine.h:188-220
+ * (generatePrimes / initFactors / generateFactors / readFactors per block):
+ * block i holds k in [i 2^21, (i + 1) 2^21), p = 3 k 2^m + 1 (mode +1) or
+ * p = 10 k -+ 1 (mode -1); the block range [i_min, i_max) of a p range is
+ * the reference's (ctsieve.h:375-377).  ct_fsieve_run emits the pairs of
+ * nblocks blocks from i_begin with p in [p_min, p_max] and b <= bMax(p) (or
+ * b <= bmax after ct_fsieve_set_bmax(s, bmax) with bmax != 0; 2^64 - 1 emits
+ * every root), re-verified on the device
+ * (the check of saveFactors, ctsieve.h:270-289), unordered; CT_ERR_NOMEM when
+ * they exceed cap (count = the size needed), at most 2^24 pairs per call. */
+typedef struct ct_fsieve ct_fsieve;
+int ct_fsieve_create(int device, int m, uint64_t p_min, uint64_t p_max, int mode, ct_fsieve **out);
+int ct_fsieve_blocks(const ct_fsieve *s, uint64_t *i_min, uint64_t *i_max);
+int ct_fsieve_set_bmax(ct_fsieve *s, uint64_t bmax);
+int ct_fsieve_run(ct_fsieve *s, uint64_t i_begin, uint64_t nblocks, uint64_t *pairs, uint64_t cap, uint64_t *count);
+int ct_fsieve_destroy(ct_fsieve *s);
+
+typedef struct
+{
+	uint64_t blocks_done;   /* blocks of [i_min, i_max) done, this call and the checkpoint's */
+	uint64_t blocks_total;  /* i_max - i_min */
+	uint64_t factors;       /* pairs written in this call */
+	double elapsed;         /* seconds, including the time restored from the checkpoint */
+	int resumed;            /* 1 if the call resumed from its checkpoint */
+	int interrupted;        /* 1 if it stopped on ct_fsieve_quit() or max_blocks */
+} ct_fsieve_stats;
+
+/* gfsieve::check (ctsieve.h:326-431), `ctsieveg <m> <p_min> <p_max> <+|->`:
+ * appends "p b" lines (sorted by p, then b, within each read) to
+ * <work_dir>/fp<ext> (mode +1) or fn<ext> (mode -1), ext =
+ * "<m>_<p_min / 10^12>_<p_max / 10^12>.txt", and keeps the text checkpoint
+ * ctx<ext> / ctx_n<ext> (blocks done, elapsed seconds) after every read,
+ * reading every j_sync blocks with j_sync adapted to record_s seconds and
+ * half of a 2^24-pair buffer (ctsieve.h:389-398).  Resumes from the
+ * checkpoint.  Stops with CT_INTERRUPTED after ct_fsieve_quit() or max_blocks
+ * blocks in this call (0 = no limit). */
+int ct_fsieve_check(int device, int m, uint64_t p_min, uint64_t p_max, int mode, const char *work_dir, double record_s,
+                    uint64_t max_blocks, ct_fsieve_stats *stats);
+/* Asks a running ct_fsieve_check to stop after its current read (the role of
+ * the reference's quit flag, ctsieve.h:166-170). */
+void ct_fsieve_quit(void);
 
 /* ---------------- files and the batch driver ---------------- */
 

@@ -4,10 +4,12 @@
 // survivors (b in ascending order, one per line: the cand_*.txt format).
 //
 // usage: ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt [dir]
+//        ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt -p p_min_pos p_min_neg
 //
 // With dir, the reference's own Sieve::write(true) then writes its state file
 // sv_<m>_<b_min>_<b_max>.dat and cand_<m>_<b_min>_<b_max>.txt into dir (the
-// byte format fixtures of the file tests).
+// byte format fixtures of the file tests).  With -p the sieve starts from the
+// given p bounds instead of the ini start (the resume state of ctsieve.cpp:790-796).
 //
 // The standard headers the reference includes are included first so that the
 // access override below only applies to the reference's own class.
@@ -35,12 +37,18 @@
 
 int main(int argc, char **argv)
 {
-	if (argc != 7 && argc != 8) { std::cerr << "usage: ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt [dir]\n"; return 1; }
+	const bool with_pmin = (argc == 10 && std::string(argv[7]) == "-p");
+	if (argc != 7 && argc != 8 && !with_pmin)
+	{
+		std::cerr << "usage: ctsieve_ref m b_min b_max p_max_pos p_max_neg out.txt [dir | -p p_min_pos p_min_neg]\n";
+		return 1;
+	}
 	const int m = std::atoi(argv[1]);
 	const uint64_t b_min = std::stoull(argv[2]), b_max = std::stoull(argv[3]);
 	const uint64_t p_max_pos = std::stoull(argv[4]), p_max_neg = std::stoull(argv[5]);
 	Sieve s(m, b_min, b_max);
 	s._p_min_pos = 3 * (1ull << m) + 1; s._p_min_neg = 11;     // the ini start (ctsieve.cpp:797)
+	if (with_pmin) { s._p_min_pos = std::stoull(argv[8]); s._p_min_neg = std::stoull(argv[9]); }
 	try
 	{
 		s.check_pos(p_max_pos);

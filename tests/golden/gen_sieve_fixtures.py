@@ -11,7 +11,10 @@ With --files it also keeps the reference's own state and candidate files
 (Sieve::write(true)) of one tiny window under tests/golden/sieve_files/, the
 byte-format fixtures of the file tests.
 
-usage: python tests/golden/gen_sieve_fixtures.py [--full] [--files]
+With --beyond it writes sieve_beyond_2e32.json: windows sieved from explicit
+p_min bounds with p across and above 2^32 (the struck b listed in full).
+
+usage: python tests/golden/gen_sieve_fixtures.py [--full] [--files] [--beyond]
 """
 import hashlib
 import json
@@ -33,14 +36,24 @@ SMALL = [
 FILES = {"m": 8, "b_min": 2, "b_max": 3001, "p_max_pos": 100000, "p_max_neg": 100000}
 FULL = {"name": "m12_config2", "m": 12, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 24) - 1,
         "p_max_pos": 1 << 32, "p_max_neg": 1 << 32}
+# p ranges that cross and lie above 2^32 (explicit p_min, the resume state)
+BEYOND = [
+    {"name": "m8_across_2e32", "m": 8, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 16) - 1,
+     "p_min_pos": (1 << 32) - (1 << 25), "p_max_pos": (1 << 32) + (1 << 27),
+     "p_min_neg": (1 << 32) - (1 << 25), "p_max_neg": (1 << 32) + (1 << 27)},
+    {"name": "m12_above_2e32", "m": 12, "b_min": 3000000, "b_max": 3000000 + (1 << 16) - 1,
+     "p_min_pos": 1 << 32, "p_max_pos": (1 << 32) + (1 << 30),
+     "p_min_neg": 1 << 32, "p_max_neg": (1 << 32) + (1 << 26)},
+]
 
 
 def run(case):
     subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref")])
     with tempfile.TemporaryDirectory() as d:
         out = os.path.join(d, "s.txt")
+        extra = ["-p", str(case["p_min_pos"]), str(case["p_min_neg"])] if "p_min_pos" in case else []
         subprocess.check_call([REF, str(case["m"]), str(case["b_min"]), str(case["b_max"]),
-                               str(case["p_max_pos"]), str(case["p_max_neg"]), out], cwd=d,
+                               str(case["p_max_pos"]), str(case["p_max_neg"]), out] + extra, cwd=d,
                               stdout=subprocess.DEVNULL)
         text = open(out).read()
     surv = [int(x) for x in text.split()]
@@ -71,6 +84,16 @@ def main():
         json.dump(dict(FILES, source="sieve/CPU/ctsieve.cpp Sieve::write(true) after check_pos + check_neg"),
                   open(os.path.join(dst, "case.json"), "w"))
         print("files", stem)
+    if "--beyond" in sys.argv:
+        cases = []
+        for c in BEYOND:
+            surv, h = run(c)
+            sv = set(surv)
+            struck = [b for b in range(c["b_min"], c["b_max"] + 1) if b not in sv]
+            cases.append(dict(c, count=len(surv), sha256=h, struck=struck))
+            print(c["name"], len(surv), "survivors,", len(struck), "struck", flush=True)
+        json.dump({"source": "sieve/CPU/ctsieve.cpp check_pos + check_neg from explicit p_min (p across 2^32)",
+                   "cases": cases}, open(os.path.join(HERE, "sieve_beyond_2e32.json"), "w"))
     if "--full" in sys.argv:
         surv, h = run(FULL)
         json.dump(dict(FULL, source="sieve/CPU/ctsieve.cpp check_pos + check_neg from the ini start",
