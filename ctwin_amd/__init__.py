@@ -38,7 +38,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free",
            "ct_sieve_factors_noalloc", "ct_engine_set_row_form", "ct_set_graphs_default",
            "ct_fsieve_create", "ct_fsieve_blocks", "ct_fsieve_set_bmax", "ct_fsieve_run", "ct_fsieve_destroy",
-           "ct_fsieve_check", "ct_fsieve_quit"]
+           "ct_fsieve_check", "ct_fsieve_quit", "ct_sieve_last_timing"]
 
 INTERRUPTED = 1
 
@@ -51,6 +51,11 @@ class CycloStats(ctypes.Structure):
 class FsieveStats(ctypes.Structure):
     _fields_ = [("blocks_done", ctypes.c_uint64), ("blocks_total", ctypes.c_uint64), ("factors", ctypes.c_uint64),
                 ("elapsed", ctypes.c_double), ("resumed", ctypes.c_int), ("interrupted", ctypes.c_int)]
+
+
+class SieveTiming(ctypes.Structure):
+    _fields_ = [("roots_neg_ms", ctypes.c_float), ("roots_neg_launches", ctypes.c_uint32), ("device_ms", ctypes.c_float),
+                ("kernels_ms", ctypes.c_float)]
 
 
 class CtError(RuntimeError):
@@ -127,6 +132,7 @@ def lib():
             "ct_fsieve_run": [vp, u64, u64, u64p, u64, c.POINTER(u64)],
             "ct_fsieve_destroy": [vp],
             "ct_fsieve_check": [i32, i32, u64, u64, i32, c.c_char_p, c.c_double, u64, c.POINTER(FsieveStats)],
+            "ct_sieve_last_timing": [i32, c.POINTER(SieveTiming)],
             "ct_engine_set_column_split": [vp, i32],
             "ct_engine_set_fused": [vp, i32],
             "ct_prp_begin": [vp, u32p, c.POINTER(vp), c.POINTER(i32), c.POINTER(i32)],
@@ -456,6 +462,17 @@ def sieve_window(m, b_min, b_max, p_max_pos, p_max_neg, mode=0, p_min_pos=None, 
         struck = np.zeros(b_max - b_min + 1, np.uint8)
     _check(lib().ct_sieve_window(device, m, b_min, b_max, p_min_pos, p_max_pos, p_min_neg, p_max_neg, mode, struck))
     return struck
+
+
+def sieve_last_timing(device=0):
+    """Device time of the last sieve_window call on the device: dict with the
+    summed duration of the N- root kernel's launches, their number, the
+    call's device span (upload to end of download) and its kernels' span
+    (between the copies), in ms."""
+    t = SieveTiming()
+    _check(lib().ct_sieve_last_timing(device, ctypes.byref(t)))
+    return {"roots_neg_ms": t.roots_neg_ms, "roots_neg_launches": int(t.roots_neg_launches), "device_ms": t.device_ms,
+            "kernels_ms": t.kernels_ms}
 
 
 def sieve_factors(m, p_min, p_max, mode, device=0, cap=1 << 20):

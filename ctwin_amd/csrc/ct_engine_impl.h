@@ -89,8 +89,11 @@ int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
 int plain_materialize(ct_engine *e);
 void choose_primes(ct_engine *e);
 bool plain_tile_ok(int ln, uint32_t w);
+// The window sieve's primes from 2^32 up: d_list holds kWindow64List words
+// and d_count one word, both owned by the caller.
+constexpr size_t kWindow64List = 2ull << 21;
 int strike_window64(hipStream_t st, int m, int mode, uint64_t k_lo, uint64_t k_hi, uint64_t p_lo, uint64_t p_hi,
-                    uint64_t b_min, uint64_t b_max, uint8_t *d_struck);
+                    uint64_t b_min, uint64_t b_max, uint8_t *d_struck, uint64_t *d_list, uint32_t *d_count);
 uint32_t plain_default_tile(int ln);
 
 }   // namespace ct

@@ -394,21 +394,12 @@ static int run_k_range(hipStream_t st, int m, int mode, uint64_t k_lo, uint64_t 
 // [b_min, b_max] with p | N+-(b) for the primes p in [p_lo, p_hi] of the k
 // range [k_lo, k_hi] (d_struck[b - b_min] = 1).
 int strike_window64(hipStream_t st, int m, int mode, uint64_t k_lo, uint64_t k_hi, uint64_t p_lo, uint64_t p_hi,
-                    uint64_t b_min, uint64_t b_max, uint8_t *d_struck)
+                    uint64_t b_min, uint64_t b_max, uint8_t *d_struck, uint64_t *d_list, uint32_t *d_count)
 {
+	static_assert(kWindow64List == 2 * kBlockK, "list size of the 64-bit prime kernel");
 	if (k_hi < k_lo || p_hi < p_lo) return CT_OK;
-	uint64_t *d_list = nullptr;
-	uint32_t *d_count = nullptr;
-	if (hipMalloc((void **)&d_list, 2 * kBlockK * sizeof(uint64_t)) != hipSuccess ||
-	    hipMalloc((void **)&d_count, sizeof(uint32_t)) != hipSuccess)
-	{
-		hipFree(d_list);
-		return fail(CT_ERR_NOMEM, "hipMalloc 64-bit prime list");
-	}
 	const strike_sink sink{d_struck, b_min, b_max};
-	const int rc = run_k_range(st, m, mode, k_lo, k_hi, p_lo, p_hi, d_list, d_count, sink);
-	hipFree(d_list); hipFree(d_count);
-	return rc;
+	return run_k_range(st, m, mode, k_lo, k_hi, p_lo, p_hi, d_list, d_count, sink);
 }
 
 }   // namespace ct

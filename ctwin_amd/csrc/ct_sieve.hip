@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ctwin.h"
@@ -325,14 +326,11 @@ __device__ __forceinline__ void strike_roots(const m32 &A, uint32_t p, uint32_t 
 	}
 }
 
-// one thread per prime: sqrt(5), y = (1 +/- sqrt 5)/2, all 2^(m-1)-th roots of y
-__global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, uint64_t b_min, uint64_t b_max,
-                            uint8_t *struck, uint2 *small, uint32_t *small_count, uint32_t small_cap,
-                            root_run *big, uint32_t *big_count, uint32_t big_cap)
+// One prime: sqrt(5), y = (1 +/- sqrt 5)/2, all 2^(m-1)-th roots of y.
+__device__ __forceinline__ void roots_neg_one(uint32_t p, int m, uint64_t b_min, uint64_t b_max, uint8_t *struck,
+                                              uint2 *small, uint32_t *small_count, uint32_t small_cap,
+                                              root_run *big, uint32_t *big_count, uint32_t big_cap)
 {
-	const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= *count) return;
-	const uint32_t p = list[idx];
 	const m32 A(p);
 	uint32_t q = p - 1; int v = 0;
 	while ((q & 1) == 0) { q >>= 1; ++v; }
@@ -378,6 +376,18 @@ __global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, 
 	}
 }
 
+// One thread per prime of list[0 .. *count), in a grid-stride loop: the grid
+// is sized from the list's capacity, so the host launches it without reading
+// the count back.
+__global__ void k_roots_neg(const uint32_t *list, const uint32_t *count, int m, uint64_t b_min, uint64_t b_max,
+                            uint8_t *struck, uint2 *small, uint32_t *small_count, uint32_t small_cap,
+                            root_run *big, uint32_t *big_count, uint32_t big_cap)
+{
+	const uint32_t n = *count;
+	for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x)
+		roots_neg_one(list[idx], m, b_min, b_max, struck, small, small_count, small_cap, big, big_count, big_cap);
+}
+
 // The queued runs of 2^e > 2^kRunInline roots, one wavefront each: lane l
 // takes the roots x u^j for j in [l 2^e / 64, (l + 1) 2^e / 64), or the single
 // root j = l of a run shorter than 64.  A grid-stride loop covers every queued
@@ -416,10 +426,68 @@ using namespace ct;
 
 #define HIPCHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) { rc = hip_fail(_e, #call); goto done; } } while (0)
 
+namespace {
+
+constexpr uint64_t kWinBlockK = 1ull << 24;            // k values per block of the candidate kernels
+constexpr uint32_t kSmallCap = 1u << 22;               // queued (p, root) pairs of small primes, all blocks
+constexpr uint32_t kBigCap = 1u << 16;                 // queued runs of roots per block
+constexpr uint32_t kRootsNegGrid = 1u << 16;           // workgroups of k_roots_neg (x 128 threads >= primes per block)
+constexpr int kTimedBlocks = 32;                       // k_roots_neg launches timed per call (p < 2^32: at most 26)
+
+// Device buffers, stream and events of the window sieve, kept per device for
+// the life of the process (about 190 MiB plus the byte map, which grows to the
+// largest window seen).  `mu` serialises calls on one device.
+struct window_ws
+{
+	std::mutex mu;
+	bool ready = false;
+	hipStream_t st = nullptr;
+	uint8_t *d_struck = nullptr;
+	size_t struck_cap = 0;
+	uint32_t *d_list = nullptr, *d_count = nullptr, *d_scount = nullptr;
+	uint2 *d_small = nullptr;
+	root_run *d_big = nullptr;
+	sieve_prime *d_sp = nullptr;
+	uint64_t *d_list64 = nullptr;
+	uint32_t *d_count64 = nullptr;
+	hipEvent_t ev[2 * kTimedBlocks + 4] = {};
+	ct_sieve_timing last = {};
+};
+
+window_ws g_win[64];
+
+int window_ws_init(window_ws &w, int device)
+{
+	if (w.ready) return CT_OK;
+	hipError_t h;
+	uint32_t ncoop = 0;
+	const std::vector<sieve_prime> &sp = sieve_primes_neg(&ncoop);
+	if ((h = hipSetDevice(device)) != hipSuccess) return hip_fail(h, "hipSetDevice");
+	if ((h = hipStreamCreate(&w.st)) != hipSuccess) return hip_fail(h, "hipStreamCreate");
+	for (hipEvent_t &e : w.ev)
+		if ((h = hipEventCreate(&e)) != hipSuccess) return hip_fail(h, "hipEventCreate");
+	if (hipMalloc((void **)&w.d_list, 2 * kWinBlockK * sizeof(uint32_t)) != hipSuccess ||
+	    hipMalloc((void **)&w.d_count, sizeof(uint32_t)) != hipSuccess ||
+	    hipMalloc((void **)&w.d_scount, 2 * sizeof(uint32_t)) != hipSuccess ||     // small pairs, queued runs
+	    hipMalloc((void **)&w.d_small, kSmallCap * sizeof(uint2)) != hipSuccess ||
+	    hipMalloc((void **)&w.d_big, kBigCap * sizeof(root_run)) != hipSuccess ||
+	    hipMalloc((void **)&w.d_sp, sp.size() * sizeof(sieve_prime)) != hipSuccess ||
+	    hipMalloc((void **)&w.d_list64, kWindow64List * sizeof(uint64_t)) != hipSuccess ||
+	    hipMalloc((void **)&w.d_count64, sizeof(uint32_t)) != hipSuccess)
+		return fail(CT_ERR_NOMEM, "hipMalloc window sieve buffers");
+	if ((h = hipMemcpy(w.d_sp, sp.data(), sp.size() * sizeof(sieve_prime), hipMemcpyHostToDevice)) != hipSuccess)
+		return hip_fail(h, "sieving primes");
+	w.ready = true;
+	return CT_OK;
+}
+
+}   // namespace
+
 int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_max_pos,
                     uint64_t p_min_neg, uint64_t p_max_neg, int mode, uint8_t *struck)
 {
 	if (!struck) return fail(CT_ERR_ARG, "struck is null");
+	if (device < 0 || device >= 64) return fail(CT_ERR_ARG, "device ordinal out of range");
 	if (m < 8 || m > 24) return fail(CT_ERR_ARG, "m must be in [8, 24] (ctsieve.cpp:841)");
 	if (b_max < b_min || b_max - b_min >= (1ull << 32)) return fail(CT_ERR_ARG, "bad b window");
 	const uint64_t P32 = 1ull << 32;
@@ -438,114 +506,126 @@ int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t 
 	const uint64_t kn64 = (P32 + 10) / 10;                                // 10 k - 1 >= 2^32
 
 	const size_t W = (size_t)(b_max - b_min + 1);
-	const uint64_t KB = 1ull << 24;                      // k values per block
-	int rc = CT_OK;
-	uint8_t *d_struck = nullptr;
-	uint32_t *d_list = nullptr, *d_count = nullptr, *d_scount = nullptr;
-	uint2 *d_small = nullptr;
-	root_run *d_big = nullptr;
-	const uint32_t big_cap = 1u << 16;
-	sieve_prime *d_sp = nullptr;
 	uint32_t ncoop = 0;
 	const std::vector<sieve_prime> &sp = sieve_primes_neg(&ncoop);
 	if (ncoop > kCoopMax) return fail(CT_ERR_UNSUPPORTED, "sieving prime table larger than the kernel holds");
-	const uint32_t small_cap = 1u << 22;
-	hipStream_t st = nullptr;
+	window_ws &w = g_win[device];
+	std::lock_guard<std::mutex> lock(w.mu);
+	int rc = window_ws_init(w, device);
+	if (rc) return rc;
+	hipStream_t st = w.st;
+	int timed = 0;
+	w.last = ct_sieve_timing{};
 	{
 		HIPCHK(hipSetDevice(device));
-		HIPCHK(hipStreamCreate(&st));
-		HIPCHK(hipMalloc((void **)&d_struck, W));
-		HIPCHK(hipMalloc((void **)&d_list, 2 * KB * sizeof(uint32_t)));
-		HIPCHK(hipMalloc((void **)&d_count, sizeof(uint32_t)));
-		HIPCHK(hipMalloc((void **)&d_scount, 2 * sizeof(uint32_t)));      // small pairs, queued runs
-		HIPCHK(hipMalloc((void **)&d_small, small_cap * sizeof(uint2)));
-		HIPCHK(hipMemcpy(d_struck, struck, W, hipMemcpyHostToDevice));
-		if (mode <= 0)
+		if (W > w.struck_cap)
 		{
-			HIPCHK(hipMalloc((void **)&d_big, big_cap * sizeof(root_run)));
-			HIPCHK(hipMalloc((void **)&d_sp, sp.size() * sizeof(sieve_prime)));
-			HIPCHK(hipMemcpy(d_sp, sp.data(), sp.size() * sizeof(sieve_prime), hipMemcpyHostToDevice));
+			HIPCHK(hipFree(w.d_struck));
+			w.d_struck = nullptr; w.struck_cap = 0;
+			HIPCHK(hipMalloc((void **)&w.d_struck, W));
+			w.struck_cap = W;
 		}
+		HIPCHK(hipEventRecord(w.ev[2 * kTimedBlocks], st));
+		HIPCHK(hipMemcpyAsync(w.d_struck, struck, W, hipMemcpyHostToDevice, st));
+		HIPCHK(hipEventRecord(w.ev[2 * kTimedBlocks + 2], st));
+		HIPCHK(hipMemsetAsync(w.d_scount, 0, 2 * sizeof(uint32_t), st));
+		// + side: one block of k covers every p < 2^32 for m >= 9 (two for m = 8)
 		if (mode >= 0 && kp_max >= kp_min)
 		{
-			for (uint64_t k0 = kp_min; k0 <= kp_max; k0 += KB)
+			for (uint64_t k0 = kp_min; k0 <= kp_max; k0 += kWinBlockK)
 			{
-				const uint64_t kc = std::min(KB, kp_max - k0 + 1);
+				const uint64_t kc = std::min(kWinBlockK, kp_max - k0 + 1);
 				if (3 * (k0 << m) + 1 >= P32) break;
-				HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
-				HIPCHK(hipMemsetAsync(d_scount, 0, 2 * sizeof(uint32_t), st));
-				hipLaunchKernelGGL(k_primes_pos, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, m, d_list, d_count);
+				HIPCHK(hipMemsetAsync(w.d_count, 0, sizeof(uint32_t), st));
+				hipLaunchKernelGGL(k_primes_pos, dim3((uint32_t)((kc + 255) / 256)), dim3(256), 0, st, k0, kc, m, w.d_list, w.d_count);
 				HIPCHK(hipGetLastError());
 				uint32_t cnt = 0;
-				HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+				HIPCHK(hipMemcpyAsync(&cnt, w.d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
 				HIPCHK(hipStreamSynchronize(st));
 				if (cnt)
 				{
-					hipLaunchKernelGGL(k_roots_pos, dim3(cnt), dim3(256), 0, st, d_list, d_count, m, b_min, b_max, d_struck,
-					                   d_small, d_scount, small_cap);
+					hipLaunchKernelGGL(k_roots_pos, dim3(cnt), dim3(256), 0, st, w.d_list, w.d_count, m, b_min, b_max, w.d_struck,
+					                   w.d_small, w.d_scount, kSmallCap);
 					HIPCHK(hipGetLastError());
-					uint32_t scnt = 0;
-					HIPCHK(hipMemcpyAsync(&scnt, d_scount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-					HIPCHK(hipStreamSynchronize(st));
-					if (scnt > small_cap) { rc = fail(CT_ERR_UNSUPPORTED, "too many small-prime roots"); goto done; }
-					if (scnt)
-					{
-						hipLaunchKernelGGL(k_strike_small, dim3(scnt), dim3(256), 0, st, d_small, d_scount, b_min, b_max, d_struck);
-						HIPCHK(hipGetLastError());
-					}
 				}
 			}
 		}
+		// - side: the blocks run back to back with no host round trip; the
+		// list, its count and the run queue are reused block after block in
+		// stream order
 		if (mode <= 0 && kn_max >= kn_min)
 		{
-			for (uint64_t k0 = kn_min; k0 <= kn_max; k0 += KB)
+			for (uint64_t k0 = kn_min; k0 <= kn_max; k0 += kWinBlockK)
 			{
-				const uint64_t kc = std::min(KB, kn_max - k0 + 1);
+				const uint64_t kc = std::min(kWinBlockK, kn_max - k0 + 1);
 				if (10 * k0 >= P32 + 1) break;
-				HIPCHK(hipMemsetAsync(d_count, 0, sizeof(uint32_t), st));
-				HIPCHK(hipMemsetAsync(d_scount, 0, 2 * sizeof(uint32_t), st));
-				hipLaunchKernelGGL(k_sieve_neg, dim3((uint32_t)((kc + kSegK - 1) / kSegK)), dim3(256), 0, st, k0, kc, d_sp,
-				                   (uint32_t)sp.size(), ncoop, d_list, d_count);
+				HIPCHK(hipMemsetAsync(w.d_count, 0, sizeof(uint32_t), st));
+				HIPCHK(hipMemsetAsync(w.d_scount + 1, 0, sizeof(uint32_t), st));
+				hipLaunchKernelGGL(k_sieve_neg, dim3((uint32_t)((kc + kSegK - 1) / kSegK)), dim3(256), 0, st, k0, kc, w.d_sp,
+				                   (uint32_t)sp.size(), ncoop, w.d_list, w.d_count);
 				HIPCHK(hipGetLastError());
-				uint32_t cnt = 0, scnt = 0;
-				HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-				HIPCHK(hipStreamSynchronize(st));
-				if (cnt)
-				{
-					hipLaunchKernelGGL(k_roots_neg, dim3((cnt + 127) / 128), dim3(128), 0, st, d_list, d_count, m, b_min, b_max,
-					                   d_struck, d_small, d_scount, small_cap, d_big, d_scount + 1, big_cap);
-					HIPCHK(hipGetLastError());
-					hipLaunchKernelGGL(k_roots_run, dim3(1024), dim3(64), 0, st, d_big, d_scount + 1, big_cap, b_min, b_max,
-					                   d_struck, d_small, d_scount, small_cap);
-					HIPCHK(hipGetLastError());
-					HIPCHK(hipMemcpyAsync(&scnt, d_scount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-					HIPCHK(hipStreamSynchronize(st));
-					if (scnt > small_cap) { rc = fail(CT_ERR_UNSUPPORTED, "too many small-prime roots"); goto done; }
-					if (scnt)
-					{
-						hipLaunchKernelGGL(k_strike_small, dim3(scnt), dim3(256), 0, st, d_small, d_scount, b_min, b_max, d_struck);
-						HIPCHK(hipGetLastError());
-					}
-				}
+				const uint32_t grid = (uint32_t)std::min<uint64_t>(kRootsNegGrid, (2 * kc + 127) / 128);
+				if (timed < kTimedBlocks) HIPCHK(hipEventRecord(w.ev[2 * timed], st));
+				hipLaunchKernelGGL(k_roots_neg, dim3(grid), dim3(128), 0, st, w.d_list, w.d_count, m, b_min, b_max,
+				                   w.d_struck, w.d_small, w.d_scount, kSmallCap, w.d_big, w.d_scount + 1, kBigCap);
+				HIPCHK(hipGetLastError());
+				if (timed < kTimedBlocks) { HIPCHK(hipEventRecord(w.ev[2 * timed + 1], st)); ++timed; }
+				hipLaunchKernelGGL(k_roots_run, dim3(1024), dim3(64), 0, st, w.d_big, w.d_scount + 1, kBigCap, b_min, b_max,
+				                   w.d_struck, w.d_small, w.d_scount, kSmallCap);
+				HIPCHK(hipGetLastError());
+			}
+		}
+		// the queued (p, root) pairs of small primes, both sides, in one launch
+		{
+			uint32_t scnt = 0;
+			HIPCHK(hipMemcpyAsync(&scnt, w.d_scount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+			HIPCHK(hipStreamSynchronize(st));
+			if (scnt > kSmallCap) { rc = fail(CT_ERR_UNSUPPORTED, "too many small-prime roots"); goto done; }
+			if (scnt)
+			{
+				hipLaunchKernelGGL(k_strike_small, dim3(scnt), dim3(256), 0, st, w.d_small, w.d_scount, b_min, b_max, w.d_struck);
+				HIPCHK(hipGetLastError());
 			}
 		}
 		// p >= 2^32 (up to the first p >= p_max, the reference's rule): the
 		// 64-bit prime and root kernels strike the same byte map
 		if (mode >= 0 && kp_max >= kp_min && kp_max >= kp64)
 		{
-			rc = strike_window64(st, m, +1, std::max(kp_min, kp64), kp_max, P32, 3 * (kp_max << m) + 1, b_min, b_max, d_struck);
+			rc = strike_window64(st, m, +1, std::max(kp_min, kp64), kp_max, P32, 3 * (kp_max << m) + 1, b_min, b_max, w.d_struck,
+			                     w.d_list64, w.d_count64);
 			if (rc) goto done;
 		}
 		if (mode <= 0 && kn_max >= kn_min && kn_max >= kn64)
 		{
-			rc = strike_window64(st, m, -1, std::max(kn_min, kn64), kn_max, P32, 10 * kn_max + 1, b_min, b_max, d_struck);
+			rc = strike_window64(st, m, -1, std::max(kn_min, kn64), kn_max, P32, 10 * kn_max + 1, b_min, b_max, w.d_struck,
+			                     w.d_list64, w.d_count64);
 			if (rc) goto done;
 		}
+		HIPCHK(hipEventRecord(w.ev[2 * kTimedBlocks + 3], st));
+		HIPCHK(hipMemcpyAsync(struck, w.d_struck, W, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipEventRecord(w.ev[2 * kTimedBlocks + 1], st));
 		HIPCHK(hipStreamSynchronize(st));
-		HIPCHK(hipMemcpy(struck, d_struck, W, hipMemcpyDeviceToHost));
+		for (int i = 0; i < timed; ++i)
+		{
+			float ms = 0;
+			HIPCHK(hipEventElapsedTime(&ms, w.ev[2 * i], w.ev[2 * i + 1]));
+			w.last.roots_neg_ms += ms;
+		}
+		w.last.roots_neg_launches = (uint32_t)timed;
+		HIPCHK(hipEventElapsedTime(&w.last.device_ms, w.ev[2 * kTimedBlocks], w.ev[2 * kTimedBlocks + 1]));
+		HIPCHK(hipEventElapsedTime(&w.last.kernels_ms, w.ev[2 * kTimedBlocks + 2], w.ev[2 * kTimedBlocks + 3]));
 	}
 done:
-	hipFree(d_struck); hipFree(d_list); hipFree(d_count); hipFree(d_scount); hipFree(d_small); hipFree(d_big); hipFree(d_sp);
-	if (st) hipStreamDestroy(st);
+	if (rc) (void)hipStreamSynchronize(st);       // nothing of this call still runs when it returns
 	return rc;
+}
+
+int ct_sieve_last_timing(int device, ct_sieve_timing *out)
+{
+	if (!out) return fail(CT_ERR_ARG, "out is null");
+	if (device < 0 || device >= 64) return fail(CT_ERR_ARG, "device ordinal out of range");
+	window_ws &w = g_win[device];
+	std::lock_guard<std::mutex> lock(w.mu);
+	*out = w.last;
+	return CT_OK;
 }

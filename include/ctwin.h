@@ -258,9 +258,27 @@ int ct_server_set_graphs(ct_server *s, int on);
  * b_max - b_min < 2^32.  Primes below 2^32 run through 32-bit kernels, the
  * others through the 64-bit ones of the factor emission.  The survivor list
  * is the set of unstruck b in ascending order (cand_*.txt of
- * ctsieve.cpp:621-626). */
+ * ctsieve.cpp:621-626).  The device buffers (about 190 MiB, plus the byte map
+ * of the largest window seen) and the stream are created by the first call on
+ * a device and kept until the process ends; calls on one device run one at a
+ * time. */
 int ct_sieve_window(int device, int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, uint64_t p_max_pos,
                     uint64_t p_min_neg, uint64_t p_max_neg, int mode, uint8_t *struck);
+
+/* Device time of the last ct_sieve_window call on a device, from HIP events
+ * recorded on its stream (a measurement aid; no counterpart in the
+ * reference): roots_neg_ms sums the launches of the N- root kernel
+ * (roots_neg_launches of them, at most 32 timed), device_ms spans the call's
+ * device work from the upload of the byte map to the end of its download,
+ * kernels_ms the part between the two copies. */
+typedef struct
+{
+	float roots_neg_ms;
+	uint32_t roots_neg_launches;
+	float device_ms;
+	float kernels_ms;
+} ct_sieve_timing;
+int ct_sieve_last_timing(int device, ct_sieve_timing *out);
 
 /* GPU factor emission: the work of `ctsieveg <n> <p_min> <p_max> <+|->`
  * (sieve/GPU/src/main.cpp:113-169, gfsieve::check, ctsieve.h:326-431, kernels

@@ -630,7 +630,10 @@ int or_jacobi(uint64_t x, uint64_t y)
 }
 
 /* Mod class, ctsieve.cpp:68-172 */
-static inline uint64_t md_mul(uint64_t a, uint64_t b, uint64_t n) { return (uint64_t)(((u128)a * b) % n); }
+/* Work counters for the sieve (a measurement aid, not part of the algorithm):
+   Montgomery products, and the products of the strike verification. */
+static uint64_t g_mp_products, g_md_products;
+static inline uint64_t md_mul(uint64_t a, uint64_t b, uint64_t n) { ++g_md_products; return (uint64_t)(((u128)a * b) % n); }
 static inline uint64_t md_sub(uint64_t a, uint64_t b, uint64_t n) { const uint64_t c = (a < b) ? n : 0; return a - b + c; }
 static inline int ilog2_64(uint64_t x) { return 63 - __builtin_clzll(x); }
 static uint64_t md_pow(uint64_t a, uint64_t e, uint64_t n)
@@ -691,7 +694,7 @@ static inline uint64_t mp_redc(const mpa *m, u128 t)
 }
 static inline uint64_t mp_add(const mpa *m, uint64_t a, uint64_t b) { const uint64_t c = (a >= m->p - b) ? m->p : 0; return a + b - c; }
 static inline uint64_t mp_sub(const mpa *m, uint64_t a, uint64_t b) { const uint64_t c = (a < b) ? m->p : 0; return a - b + c; }
-static inline uint64_t mp_mul(const mpa *m, uint64_t a, uint64_t b) { return mp_redc(m, (u128)a * b); }
+static inline uint64_t mp_mul(const mpa *m, uint64_t a, uint64_t b) { ++g_mp_products; return mp_redc(m, (u128)a * b); }
 static inline uint64_t mp_half(const mpa *m, uint64_t a) { return (a % 2 == 0) ? a / 2 : a / 2 + m->p / 2 + 1; }
 static inline uint64_t mp_toint(const mpa *m, uint64_t r) { const uint64_t mp = mp_mulhi(r * m->q, m->p); return (mp != 0) ? m->p - mp : 0; }
 static uint64_t mp_pow(const mpa *m, uint64_t a, uint64_t e)
@@ -1128,4 +1131,11 @@ int or_sieve_factors(int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *
 	}
 	*count = o.count;
 	return o.err ? -1 : 0;
+}
+
+/* Reads (and with reset != 0 clears) the sieve's work counters. */
+void or_sieve_products(int reset, uint64_t *mont, uint64_t *verify)
+{
+	*mont = g_mp_products; *verify = g_md_products;
+	if (reset) { g_mp_products = 0; g_md_products = 0; }
 }

@@ -141,6 +141,12 @@ int or_sieve_window(int m, uint64_t b_min, uint64_t b_max, uint64_t p_min_pos, u
  * *count gets the total even when it exceeds cap. */
 int or_sieve_factors(int m, uint64_t p_min, uint64_t p_max, int mode, uint64_t *pairs, uint64_t cap, uint64_t *count);
 
+/* Work counters of the two functions above: 64-bit Montgomery products
+ * (MpArith::mul, ctsieve.cpp:175-395 / kernel.cl's Montgomery mul) and the
+ * products of the strike and pair verifications.  reset != 0 clears them after
+ * reading.  Not thread-safe: a measurement aid for single-threaded runs. */
+void or_sieve_products(int reset, uint64_t *mont, uint64_t *verify);
+
 #ifdef __cplusplus
 }
 #endif

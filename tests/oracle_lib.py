@@ -58,6 +58,7 @@ def lib():
         L.or_sieve_window.restype = c.c_int
         L.or_sieve_factors.argtypes = [c.c_int, c.c_uint64, c.c_uint64, c.c_int, u64p, c.c_uint64, c.POINTER(c.c_uint64)]
         L.or_sieve_factors.restype = c.c_int
+        L.or_sieve_products.argtypes = [c.c_int, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64)]
         L.or_powm.argtypes = [c.c_char_p, c.c_size_t, c.c_char_p, c.c_size_t, c.c_char_p, c.c_size_t,
                               c.c_char_p, c.c_size_t]
         L.or_powm.restype = c.c_int
@@ -173,6 +174,13 @@ def sieve_window(m, b_min, b_max, p_max_pos, p_max_neg, mode=0, p_min_pos=None, 
     rc = lib().or_sieve_window(m, b_min, b_max, p_min_pos, p_max_pos, p_min_neg, p_max_neg, mode, struck)
     assert rc == 0
     return struck
+
+
+def sieve_products(reset=True):
+    """(Montgomery products, verification products) counted since the last reset."""
+    a, v = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().or_sieve_products(1 if reset else 0, ctypes.byref(a), ctypes.byref(v))
+    return a.value, v.value
 
 
 def sieve_factors(m, p_min, p_max, mode, cap=1 << 20):
