@@ -3,8 +3,10 @@
 Workload (the batched configuration): 256 candidates N = b^n - b^(n/2) + 1,
 n = 2^14 digits, b uniform in [2^18, 2^19) from a fixed seed, run through the
 base-2 PRP loop (x <- x^2 * 2^bit, each lane from its own top exponent bit) in
-one persistent launch of the autotuned engine; a step is one squaring of all
-256 candidates, and every lane is checked against GMP after the timed run.  Beside it: the plain one-candidate engine
+one persistent launch of the autotuned engine; a step is a chunk of 250
+squarings of all 256 candidates (the default 20 steps are the configuration's
+5000 squarings per candidate, timed as one launch), and every lane is checked
+against GMP after the timed run.  Beside it: the plain one-candidate engine
 at n = 2^14 (batched / plain ratio) and the single large candidate at n = 2^21
 (b = 65536 + 2u), with its HBM roofline, and the oracle timed on the host CPU.
 
@@ -24,6 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_T = 39.3          # 256 CU x 64 lanes x 2.4 GHz, 32-bit ops (mul_hi/mul_lo full rate on gfx950)
 SEED = 20261203
+SQ_PER_STEP = 250           # squarings of every batched candidate per bench step
 TUNE_NAMES = ("resident", "multipass_w16", "multipass_w32", "multipass_w16_split", "multipass_w32_split",
               "multipass_w16_fused", "multipass_w32_fused", "multipass_w16_fused_split", "multipass_w32_fused_split",
               "multipass_best_rows_in_sequence")
@@ -52,7 +55,7 @@ TRAFFIC_N2E21_SQUARING = (33170 * 1.994 + 16384 + 8633 * 1.994 + 16904) * 1024  
 # an arithmetic instruction).  One slot takes a SIMD 4 clocks, so the issue
 # ceiling is 1024 SIMDs x 2.4 GHz / 4 = 614.4 G slots/s.
 VALU_SLOT_PEAK_G = 1024 * 2.4 / 4
-VALU_SLOTS_PER_STEP = (901792742 - 18354182) / 49                # k_res_steps<14, 2>, 256 lanes: 50- minus 1-squaring launch
+VALU_SLOTS_PER_SQUARING = (901792742 - 18354182) / 49                # k_res_steps<14, 2>, 256 lanes: 50- minus 1-squaring launch
 VALU_SLOTS_N2E21_SQUARING = 6934861 + 6451372                   # k_rowB<21, 8> + k_colCA2<21, 8, 32>
 SIEVE_ROOTS_NEG_SLOTS = 7505592064 - 1826200320                 # k_roots_neg, config 2, its 26 launches
 # k_roots_neg per launch on config 2: the prime list it reads (FETCH x 1.994)
@@ -132,8 +135,10 @@ def batch_parity(ct, t, bs, done, i0):
 
 
 def run_batched(ct, steps, warmup, csize=0):
-    """256 lanes, each from its own top exponent bit: lane l's exponent is
-    shifted left so that its top bit sits at the common index i0, so every
+    """`warmup` and `steps` count squarings here (the caller converts bench
+    steps of SQ_PER_STEP squarings).  256 lanes, each from its own top
+    exponent bit: lane l's exponent is shifted left so that its top bit sits
+    at the common index i0, so every
     lane carries a non-trivial residue from the first squaring (in a lockstep
     batch from a common i0, lanes with a shorter exponent square x = 1 for up
     to thousands of steps).  After `steps` timed squarings lane l holds
@@ -424,8 +429,9 @@ def cpu_baseline(ct, budget_s=5.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps of the batched workload, each SQ_PER_STEP squarings of all 256 candidates")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--big-steps", type=int, default=10000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -464,8 +470,10 @@ def main():
         print("[bench] profile run done (copy kernel moves 8388608 bytes each way)", flush=True)
         return
 
-    sq_s, ms, i0, bs, kind, tuning, parity = run_batched(ct, args.steps, args.warmup)
-    print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps); "
+    batch_sq = args.steps * SQ_PER_STEP
+    sq_s, ms, i0, bs, kind, tuning, parity = run_batched(ct, batch_sq, max(1, args.warmup * SQ_PER_STEP))
+    print(f"[bench] batched n=2^14 x256: {sq_s:.4g} squarings/s ({ms:.1f} ms for {args.steps} steps of "
+          f"{SQ_PER_STEP} squarings); "
           f"parity {parity}", flush=True)
     plain_sq_s, plain_ms, plain_cfg = run_plain(ct, 14, bs[0], args.plain_steps, 5, graphs=not args.no_graphs)
     print(f"[bench] plain n=2^14 x1: {plain_sq_s:.4g} squarings/s", flush=True)
@@ -488,7 +496,7 @@ def main():
     n = 1 << 14
     per_cand = prp_work(i0)
     valu_achieved = mul32_alg(n, 2) * sq_s / 1e12
-    valu_issue = VALU_SLOTS_PER_STEP / (VALU_SLOT_PEAK_G * 1e9) / (ms / 1e3 / args.steps)
+    valu_issue = VALU_SLOTS_PER_SQUARING / (VALU_SLOT_PEAK_G * 1e9) / (ms / 1e3 / batch_sq)
     big_n = 1 << 21
     hbm_achieved = bytes_alg(big_n) * big_sq_s / 1e9
     out = {
@@ -507,7 +515,8 @@ def main():
         "config": {"workload": "batched: 256 candidates b^(2^14) - b^(2^13) + 1, b in [2^18, 2^19), "
                                "base-2 PRP loop from each lane's own top exponent bit (every residue "
                                "non-trivial), one persistent launch of the autotuned engine",
-                   "log2_n": 14, "candidates": 256, "seed": SEED,
+                   "log2_n": 14, "candidates": 256, "seed": SEED, "squarings_per_step": SQ_PER_STEP,
+                   "squarings_timed": batch_sq,
                    "engine": "resident" if kind == ct.KIND_RESIDENT else "multipass",
                    "autotuner_ms_per_step": dict(zip(TUNE_NAMES, tuning[3]))},
         "candidates_per_hour": 3600.0 * sq_s / per_cand,

@@ -27,6 +27,17 @@ namespace ct {
 
 constexpr int E = 16;
 
+// Store of a residue that the next launch reads, possibly on another XCD: a
+// system-scope relaxed store (gfx950 `sc0 sc1`) writes through the L2, so the
+// plane drains to memory while the launch still computes instead of in the
+// write-back of dirty L2 lines at its end (measured at n = 2^21: 1.4 us less
+// per row-pass launch).  The value and the order of the stores are those of a
+// plain store; the launch boundary orders them before the next launch's reads.
+__device__ __forceinline__ void store_through(uint32_t *p, uint32_t v)
+{
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <int LNL>
 struct geo
 {

@@ -67,6 +67,17 @@ struct colgeo
 	static constexpr int passes = RL / 4;
 };
 
+// Workgroup b -> column group, XCD-aware.  Consecutive workgroups go to the
+// 8 XCDs in turn; with a tile of 16 columns (64-byte row segments) two
+// neighbouring groups share every 128-byte line, so neighbouring groups are
+// given to workgroups of one XCD: b -> (b mod 8) total/8 + b div 8 (a
+// bijection when 8 divides total) hands each XCD a contiguous range of
+// columns, whose lines its L2 then fetches once.
+__device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t total)
+{
+	return (total % 8u == 0u) ? (b % 8u) * (total / 8u) + b / 8u : b;
+}
+
 // column forward passes 0..passes-1 (transposes between them only)
 // Slot of chunk g (16 digits; row g / cpr, chunk g % cpr of the row, cpr =
 // chunks per row) in a lane's carry array: chunk-column major, so that the
@@ -129,7 +140,7 @@ __device__ __forceinline__ void colA_prime_all(const int32_t *D, uint32_t *X, ui
 		else col_fwd<RL, 2, 0>(u, Xc, twf, tt, w0);
 		uint32_t *out = res_lane + P * plane + c0 + cw;
 #pragma unroll
-		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, La>(tt, q) * C] = u[q];
+		for (int q = 0; q < E; ++q) store_through(&out[(size_t)elem<RL, La>(tt, q) * C], u[q]);
 	}
 }
 
@@ -186,7 +197,7 @@ __device__ __forceinline__ void colA_all_shared(const int32_t *D, uint32_t *cons
 	{
 		uint32_t *out = res_lane + P * plane + c0 + cw;
 #pragma unroll
-		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, 1>(tt, q) * C] = u[P][q];
+		for (int q = 0; q < E; ++q) store_through(&out[(size_t)elem<RL, 1>(tt, q) * C], u[P][q]);
 	}
 }
 
@@ -204,7 +215,7 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colA(plain_arg
 	__shared__ int32_t D[G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
 	__shared__ uint32_t Y[(NP == 3 && G::passes == 2 && !SPLIT) ? G::W * G::RP : 1];    // third transpose plane
-	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
+	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = xcd_group(blockIdx.x, gridDim.x) * G::W;
 	if (A.lane_i && A.lane_i[lane] < 0) return;
 	const int32_t *x = (operand ? A.xb : A.x) + (size_t)lane * nd;
 	uint32_t *res = (operand ? A.res2 : A.res) + (size_t)lane * n;
@@ -278,7 +289,7 @@ __device__ __forceinline__ void rowB_prime(plain_args &A, uint32_t *lds, uint32_
 	}
 	final_inv<CL, P, false, MUL, true>(u, v, lds, twf, twi, base, t, F);
 #pragma unroll
-	for (int q = 0; q < E; ++q) row[t + q * T] = u[q];
+	for (int q = 0; q < E; ++q) store_through(&row[t + q * T], u[q]);
 }
 
 template <int LN, int RL, bool MUL>
@@ -319,7 +330,7 @@ __device__ __forceinline__ void rowS_prime(plain_args &A, uint32_t *lds, uint32_
 	final_inv<CL, P, false, false, true>(u, dummy, lds, twf, twi, R + r, t, F);
 	uint32_t *row = A.res + P * plane + (size_t)lane * n + (size_t)r * C;
 #pragma unroll
-	for (int q = 0; q < E; ++q) row[t + q * T] = u[q];
+	for (int q = 0; q < E; ++q) store_through(&row[t + q * T], u[q]);
 	if constexpr (P + 1 < NP)
 	{
 		__syncthreads();                 // every wave is done with this prime's LDS transposes
@@ -524,7 +535,7 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 	constexpr uint32_t nd = MINUS ? n / 2 : n, nc = nd / 16, RD = MINUS ? G::R / 2 : G::R;
 	__shared__ uint32_t S[NP][G::W * G::RP];
 	__shared__ uint32_t X[G::W * G::RP];
-	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = blockIdx.x * G::W;
+	const uint32_t t = threadIdx.x, lane = blockIdx.y, c0 = xcd_group(blockIdx.x, gridDim.x) * G::W;
 	if (A.lane_i && A.lane_i[lane] < 0) return;
 	int32_t dig[E];
 	int64_t f;
@@ -549,7 +560,8 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colC(plain_arg
 // holds it (group 0 waits for the last group, whose rows feed the next row's
 // first chunk and, with X^n = X^(n/2) - 1, chunks 0 and n/32).  Every group
 // publishes before it waits, so the waits end once every group has started:
-// when the grid fits the device at once its workgroups are named by blockIdx;
+// when the grid fits the device at once its workgroups are named by their
+// block index (through xcd_group);
 // otherwise (use_ticket) each takes a ticket (ctl[1], one atomic add; the
 // holder of the last ticket resets the counter) and the ticket names its
 // (lane, column group), so a group waits only for groups that started before
@@ -569,7 +581,7 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 	__shared__ int64_t fl[G::threads];
 	__shared__ uint32_t ticket;
 	const uint32_t t = threadIdx.x;
-	uint32_t grp = blockIdx.x;
+	uint32_t grp = xcd_group(blockIdx.x, gridDim.x);
 	if (use_ticket)
 	{
 		if (t == 0)
@@ -582,10 +594,11 @@ __global__ __launch_bounds__((colgeo<LN, RL, W>::threads)) void k_colCA(plain_ar
 		__syncthreads();
 		grp = ticket;
 	}
-	// unsigned: the epoch wraps after 2^32 launches (flags are only compared for equality)
-	const int32_t epoch = (int32_t)((uint32_t)__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
-	                                (uint32_t)A.step + 1u);
 	const uint32_t lane = grp / GW, j = grp % GW, c0 = j * G::W;
+	// ctl[2] changes only between launches (k_epoch_add), so a plain load (one
+	// scalar load per wavefront) reads it; unsigned: the epoch wraps after 2^32
+	// launches (flags are only compared for equality)
+	const int32_t epoch = (int32_t)((uint32_t)ctl[2] + (uint32_t)A.step + 1u);
 	// an idle lane (per-lane runs): all its groups leave; waits stay within a lane
 	if (A.lane_i && A.lane_i[lane] < 0) return;
 	int32_t dig[E];
@@ -671,7 +684,7 @@ __global__ __launch_bounds__((2 * colgeo<LN, RL, W>::threads)) void k_colCA2(pla
 	__shared__ uint32_t ticket;
 	const uint32_t tf = threadIdx.x;
 	const uint32_t P = tf / TH, t = tf % TH;             // P is uniform over a wavefront
-	uint32_t grp = blockIdx.x;
+	uint32_t grp = xcd_group(blockIdx.x, gridDim.x);
 	if (use_ticket)
 	{
 		if (tf == 0)
@@ -684,9 +697,11 @@ __global__ __launch_bounds__((2 * colgeo<LN, RL, W>::threads)) void k_colCA2(pla
 		__syncthreads();
 		grp = ticket;
 	}
-	const int32_t epoch = (int32_t)((uint32_t)__hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) +
-	                                (uint32_t)A.step + 1u);
 	const uint32_t lane = grp / GW, j = grp % GW, c0 = j * G::W;
+	// ctl[2] changes only between launches (k_epoch_add), so a plain load (one
+	// scalar load per wavefront) reads it; unsigned: the epoch wraps after 2^32
+	// launches (flags are only compared for equality)
+	const int32_t epoch = (int32_t)((uint32_t)ctl[2] + (uint32_t)A.step + 1u);
 	if (A.lane_i && A.lane_i[lane] < 0) return;
 	const uint32_t cw = t % G::W, tt = t / G::W;
 	const size_t plane = (size_t)A.lanes * n;
@@ -793,7 +808,7 @@ __global__ __launch_bounds__((2 * colgeo<LN, RL, W>::threads)) void k_colCA2(pla
 		else col_fwd_hi<RL, 1>(u, twf, tt);
 		uint32_t *out = A.res + P * plane + (size_t)lane * n + c0 + cw;
 #pragma unroll
-		for (int q = 0; q < E; ++q) out[(size_t)elem<RL, 1>(tt, q) * C] = u[q];
+		for (int q = 0; q < E; ++q) store_through(&out[(size_t)elem<RL, 1>(tt, q) * C], u[q]);
 	}
 }
 
