@@ -452,6 +452,84 @@ __device__ __forceinline__ void final_inv(uint32_t (&u)[E], uint32_t (&vv)[E], u
 	inv_chain<LNL, P, a - 1, PHI6, PF>(u, lds, twi, base, tt, c, M);
 }
 
+// ---- primes 0 and 1 of one instance, pass by pass ---------------------------
+//
+// The two-prime squaring runs both primes' passes in turn at every level
+// instead of one whole transform after the other: prime 0 exchanges through
+// l0 and prime 1 through l1, the two share each workgroup barrier, and the LDS
+// traffic of one prime's exchange is in flight while the other prime's pass
+// computes.  Every element sees the same butterflies as in the single-prime
+// chains (roots read where they are used), so the residues are identical.
+
+template <int LNL, int FROM, int TO>
+__device__ __forceinline__ void xchg2(uint32_t (&u0)[E], uint32_t (&u1)[E], uint32_t *l0, uint32_t *l1, uint32_t tt)
+{
+	uint32_t *w0 = l0 + lbase<LNL, FROM>(tt), *w1 = l1 + lbase<LNL, FROM>(tt);
+	uint32_t *r0 = l0 + lbase<LNL, TO>(tt), *r1 = l1 + lbase<LNL, TO>(tt);
+	constexpr bool local = wave_local<LNL, FROM>() && wave_local<LNL, TO>();
+	if constexpr (local) __builtin_amdgcn_wave_barrier();
+	else __syncthreads();
+#pragma unroll
+	for (int q = 0; q < E; ++q) w0[loff<LNL, FROM>(q)] = u0[q];
+#pragma unroll
+	for (int q = 0; q < E; ++q) w1[loff<LNL, FROM>(q)] = u1[q];
+	if constexpr (local) __builtin_amdgcn_wave_barrier();
+	else __syncthreads();
+#pragma unroll
+	for (int q = 0; q < E; ++q) u0[q] = r0[loff<LNL, TO>(q)];
+#pragma unroll
+	for (int q = 0; q < E; ++q) u1[q] = r1[loff<LNL, TO>(q)];
+	if constexpr (local) __builtin_amdgcn_wave_barrier();
+}
+
+template <int LNL, int L, bool PHI6>
+__device__ __forceinline__ void fwd_chain2(uint32_t (&u0)[E], uint32_t (&u1)[E], uint32_t *l0, uint32_t *l1,
+                                           const tw_entry *twf0, const tw_entry *twf1, uint32_t base, uint32_t tt)
+{
+	if constexpr (L < geo<LNL>::a)
+	{
+		fwd_pass<0, PHI6 && L == 0>(u0, lazy_ptw<LNL, L>(twf0, base, tt));
+		fwd_pass<1, PHI6 && L == 0>(u1, lazy_ptw<LNL, L>(twf1, base, tt));
+		xchg2<LNL, L, L + 1>(u0, u1, l0, l1, tt);
+		fwd_chain2<LNL, L + 1, PHI6>(u0, u1, l0, l1, twf0, twf1, base, tt);
+	}
+}
+
+template <int LNL, int L, bool PHI6>
+__device__ __forceinline__ void inv_chain2(uint32_t (&u0)[E], uint32_t (&u1)[E], uint32_t *l0, uint32_t *l1,
+                                           const tw_entry *twi0, const tw_entry *twi1, uint32_t base, uint32_t tt,
+                                           const mtw &M0, const mtw &M1)
+{
+	if constexpr (L >= 0)
+	{
+		xchg2<LNL, L + 1, L>(u0, u1, l0, l1, tt);
+		inv_pass<0, PHI6 && L == 0>(u0, lazy_ptw<LNL, L>(twi0, base, tt), M0);
+		inv_pass<1, PHI6 && L == 0>(u1, lazy_ptw<LNL, L>(twi1, base, tt), M1);
+		inv_chain2<LNL, L - 1, PHI6>(u0, u1, l0, l1, twi0, twi1, base, tt, M0, M1);
+	}
+}
+
+// Forward passes, final pass (pointwise square) and inverse passes of primes 0
+// and 1 (tw: the table of prime 0; prime 1's follows at tw + 2 n).
+template <int LNL, bool PHI6>
+__device__ __forceinline__ void square2(uint32_t (&u0)[E], uint32_t (&u1)[E], uint32_t *l0, uint32_t *l1,
+                                        const tw_entry *tw, uint32_t base, uint32_t tt)
+{
+	constexpr uint32_t n = geo<LNL>::n;
+	const tw_entry *twf0 = tw, *twi0 = tw + n, *twf1 = tw + 2 * n, *twi1 = tw + 3 * n;
+	fwd_chain2<LNL, 0, PHI6>(u0, u1, l0, l1, twf0, twf1, base, tt);
+	mtw M0, M1;
+	if constexpr (PHI6) { M0 = load_mtw(twi0); M1 = load_mtw(twi1); }
+	uint32_t none[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q) none[q] = 0;
+	ftw_lazy<LNL> F0; F0.twf = twf0; F0.twi = twi0; F0.base = base; F0.k0 = E * tt;
+	ftw_lazy<LNL> F1; F1.twf = twf1; F1.twi = twi1; F1.base = base; F1.k0 = E * tt;
+	final_pass<LNL, 0, false>(u0, none, F0);
+	final_pass<LNL, 1, false>(u1, none, F1);
+	inv_chain2<LNL, geo<LNL>::a - 1, PHI6>(u0, u1, l0, l1, twi0, twi1, base, tt, M0, M1);
+}
+
 // Garner CRT of the three engine residues to a signed 96-bit integer in
 // (-M/2, M/2] (the role of garner3, kernel.cl:451-460).
 __device__ __forceinline__ i96 garner(uint32_t r1, uint32_t r2, uint32_t r3, const garner_consts &g)

@@ -81,6 +81,50 @@ __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[
 	}
 }
 
+// Two-prime squaring of the digits in D (layout 0): both primes' passes in
+// turn at every level (square2), prime 1 exchanging through D itself, which is
+// free once every thread has read its digits (each thread reads only its own
+// layout-0 words, and the first exchange into D starts with a barrier, or is
+// wave-local and reaches only the wave's own words).  Residues out as in
+// transform (layout 0; MINUS: folded).
+template <int LN, bool MINUS>
+__device__ __forceinline__ void transform2(int32_t *D, uint32_t (&out0)[E], uint32_t (&out1)[E], uint32_t *lds,
+                                           const tw_entry *tw, uint32_t t)
+{
+	constexpr uint32_t p0 = kP[0], p1 = kP[1];
+	const int32_t *dr = D + lbase<LN, 0>(t);
+#pragma unroll
+	for (int k = 0; k < E; ++k)
+	{
+		const int32_t d = dr[loff<LN, 0>(k)];
+		out0[k] = from_signed_np<2>(d, p0);
+		out1[k] = from_signed_np<2>(d, p1);
+	}
+	square2<LN, true>(out0, out1, lds, reinterpret_cast<uint32_t *>(D), tw, 1u, t);
+	if constexpr (MINUS)
+	{
+		uint32_t f0[E], f1[E];
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+		{
+			f0[q] = red1p(red1p(out0[q] + out0[q + 8], p0) + out0[q + 12], p0);
+			f1[q] = red1p(red1p(out1[q] + out1[q + 8], p1) + out1[q + 12], p1);
+		}
+#pragma unroll
+		for (int q = 4; q < 8; ++q)
+		{
+			f0[q] = red1p(red1p(out0[q] + out0[q + 4], p0) + red1p(out0[q + 8] + out0[q + 8], p0), p0);
+			f1[q] = red1p(red1p(out1[q] + out1[q + 4], p1) + red1p(out1[q + 8] + out1[q + 8], p1), p1);
+		}
+#pragma unroll
+		for (int q = 0; q < E; ++q)
+		{
+			out0[q] = (q < 8) ? f0[q] : 0u;
+			out1[q] = (q < 8) ? f1[q] : 0u;
+		}
+	}
+}
+
 // One squaring (MUL = false, x <- x^2 * base^dup) or product (MUL = true,
 // x <- x * y) with CRT and the chunked carry, over NP primes (NP = 2 when every
 // coefficient is known to be below p1 p2 / 2, else 3).  x lives in LDS region
@@ -93,9 +137,13 @@ __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool du
 	constexpr uint32_t T = geo<LN>::T;
 	constexpr uint32_t TC = MINUS ? T / 2 : T;          // threads that own a chunk of digits
 	uint32_t r0[E], r1[E], r2[E];
-	transform<LN, 0, NP, MUL, MINUS>(D, yd, r0, lds, tw, t);
-	transform<LN, 1, NP, MUL, MINUS>(D, yd, r1, lds, tw, t);
-	if (NP == 3) transform<LN, 2, NP, MUL, MINUS>(D, yd, r2, lds, tw, t);
+	if constexpr (NP == 2 && !MUL) transform2<LN, MINUS>(D, r0, r1, lds, tw, t);
+	else
+	{
+		transform<LN, 0, NP, MUL, MINUS>(D, yd, r0, lds, tw, t);
+		transform<LN, 1, NP, MUL, MINUS>(D, yd, r1, lds, tw, t);
+		if (NP == 3) transform<LN, 2, NP, MUL, MINUS>(D, yd, r2, lds, tw, t);
+	}
 	// to the carry layout: primes 0 and 1 share one transpose, prime 1 through
 	// D (every read of the digits came before the first barrier of prime 1's
 	// transform; each thread later writes its carried digits to exactly the
