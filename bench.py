@@ -29,7 +29,7 @@ SEED = 20261203
 SQ_PER_STEP = 250           # squarings of every batched candidate per bench step
 TUNE_NAMES = ("resident", "multipass_w16", "multipass_w32", "multipass_w16_split", "multipass_w32_split",
               "multipass_w16_fused", "multipass_w32_fused", "multipass_w16_fused_split", "multipass_w32_fused_split",
-              "multipass_best_rows_in_sequence")
+              "multipass_best_row_primes_in_sequence", "multipass_best_row_primes_in_slices")
 
 
 def mul32_alg(n, primes=3):
@@ -198,11 +198,11 @@ def run_plain(ct, ln, b, steps, warmup, graphs=True, config=None):
     t.timer_start()
     t.run(i, i - steps + 1)
     ms = t.timer_stop()
-    w, split, fused, _, row_seq = t.tuning()
+    w, split, fused, _, row_form = t.tuning()
     t.read_state(0)                 # the read checks the fused pass's carry exchanges
     t.close()
     return steps / (ms / 1e3), ms, {"column_tile": w, "split_pass_a": split, "fused_pass_ca": fused,
-                                    "row_primes_in_sequence": row_seq}
+                                    "row_form": row_form}
 
 
 def run_sieve_config2(ct):

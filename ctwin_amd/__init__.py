@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libctwin_amd.so")
 
 KIND_AUTO, KIND_RESIDENT, KIND_PLAIN = 0, 1, 2
-TUNE_TIMINGS = 10                    # CT_TUNE_TIMINGS
+TUNE_TIMINGS = 11                    # CT_TUNE_TIMINGS
 
 STATUS = {0: "CT_OK", -1: "CT_ERR_ARG", -2: "CT_ERR_HIP", -3: "CT_ERR_NOMEM", -4: "CT_ERR_UNSUPPORTED",
           -5: "CT_ERR_GETINT", -6: "CT_ERR_COMPUTE", -7: "CT_ERR_GERBICZ"}
@@ -259,18 +259,20 @@ class Transform:
         """(column tile, split of column pass A, fused column pass, ms per
         squaring for {resident, multipass W=16, W=32, W=16 split, W=32 split,
         W=16 fused, W=32 fused, W=16 fused split, W=32 fused split, the
-        fastest with the sequential row form}; -1 = not timed, sequential row
-        form).  The row form is last so that the first four entries keep their
-        meaning."""
-        w, sp, fu, rs = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        fastest with row form 1, with row form 2}; -1 = not timed, row form
+        0/1/2).  The row form is last so that the first four entries keep
+        their meaning."""
+        w, sp, fu, rf = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = np.zeros(TUNE_TIMINGS, np.float32)
-        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ctypes.byref(fu), ctypes.byref(rs), ms))
-        return int(w.value), bool(sp.value), bool(fu.value), [float(x) for x in ms], bool(rs.value)
+        _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ctypes.byref(fu), ctypes.byref(rf), ms))
+        return int(w.value), bool(sp.value), bool(fu.value), [float(x) for x in ms], int(rf.value)
 
-    def set_row_form(self, seq):
-        """Multipass engine: row pass per (row, prime) workgroup (0) or one
-        workgroup per row transforming its primes in sequence (1)."""
-        _check(lib().ct_engine_set_row_form(self.h, int(bool(seq))))
+    def set_row_form(self, form):
+        """Multipass engine: row pass as one workgroup per (row, prime) (0),
+        one workgroup per row transforming its primes in sequence (1), or one
+        workgroup per row with a slice of threads per prime (2; used where
+        the slices are whole wavefronts within 1024 threads, else 0)."""
+        _check(lib().ct_engine_set_row_form(self.h, int(form)))
 
     def set_fused(self, on):
         """Multipass engine (N+, n >= 2^13): column pass C fused with the next

@@ -120,7 +120,7 @@ namespace {
 struct tune_key { int device; uint32_t ln, lanes; int kind, sign, np;
 	bool operator<(const tune_key &o) const
 	{ return std::tie(device, ln, lanes, kind, sign, np) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign, o.np); } };
-struct tune_val { int kind; uint32_t col_w; bool split, fused, row_seq; float ms[CT_TUNE_TIMINGS]; };
+struct tune_val { int kind; uint32_t col_w; bool split, fused; int row_form; float ms[CT_TUNE_TIMINGS]; };
 std::mutex g_tune_mu;
 std::map<tune_key, tune_val> g_tune;
 
@@ -160,7 +160,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, int
 		auto it = g_tune.find(key);
 		if (it != g_tune.end()) { *out = it->second; return CT_OK; }
 	}
-	tune_val v{CT_KIND_AUTO, 0, false, false, false, {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+	tune_val v{CT_KIND_AUTO, 0, false, false, 0, {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
 	float best = 1e30f;
 	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && tln <= 14);
 	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && tln >= (sign < 0 ? 11u : 10u));
@@ -212,15 +212,23 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, int
 			if (ms < best) { best = ms; v.kind = CT_KIND_PLAIN; v.col_w = plain_default_tile((int)tln); }
 		}
 		if (v.kind == CT_KIND_PLAIN)
-		{   // the best multipass configuration with the row pass's primes in sequence
-			e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->row_seq = true; e->fused_w = 0;
-			float ms = time_squarings(e, np);
-			if (ms >= 0 && v.fused && plain_materialize(e) != CT_OK) ms = -1;
-			e->row_seq = false; e->use_fused = false;
-			if (ms >= 0)
+		{   // the best multipass configuration with the other row-pass workgroup
+			// forms: one workgroup per row with its primes in sequence (1), or in
+			// slices of one workgroup (2, where the slices fit)
+			const uint32_t T = (1u << (tln - plain_rows_log2((int)tln))) / 16;
+			const tune_val base = v;
+			for (int form = 1; form <= 2; ++form)
 			{
-				v.ms[9] = ms;
-				if (ms < best) { best = ms; v.row_seq = true; }
+				if (form == 2 && (T % 64 != 0 || (uint32_t)np * T > 1024)) continue;
+				e->col_w = base.col_w; e->split_a = base.split; e->use_fused = base.fused; e->row_form = form; e->fused_w = 0;
+				float ms = time_squarings(e, np);
+				if (ms >= 0 && base.fused && plain_materialize(e) != CT_OK) ms = -1;
+				e->row_form = 0; e->use_fused = false;
+				if (ms >= 0)
+				{
+					v.ms[8 + form] = ms;
+					if (ms < best) { best = ms; v.row_form = form; }
+				}
 			}
 		}
 		ct_engine_destroy(e);
@@ -257,7 +265,7 @@ int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t cs
 	{
 		std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
 		(*out)->use_fused = v.fused;
-		(*out)->row_seq = v.row_seq;
+		(*out)->row_form = v.row_form;
 		(*out)->tuned = true;
 		(*out)->tuned_np = 3;
 	}
@@ -289,11 +297,12 @@ int ct_engine_set_fused(ct_engine *e, int on)
 	return CT_OK;
 }
 
-int ct_engine_set_row_form(ct_engine *e, int seq)
+int ct_engine_set_row_form(ct_engine *e, int form)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
 	if (e->kind != CT_KIND_PLAIN) return fail(CT_ERR_UNSUPPORTED, "the row pass belongs to the multipass engine");
-	e->row_seq = (seq != 0);
+	if (form < 0 || form > 2) return fail(CT_ERR_ARG, "row form must be 0, 1 or 2");
+	e->row_form = form;
 	return CT_OK;
 }
 
@@ -305,13 +314,13 @@ int ct_engine_set_column_split(ct_engine *e, int on)
 	return CT_OK;
 }
 
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, int *row_seq, float *ms)
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, int *row_form, float *ms)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
 	if (col_w) *col_w = (e->kind == CT_KIND_PLAIN) ? e->col_w : 0;
 	if (split) *split = (e->kind == CT_KIND_PLAIN && e->split_a) ? 1 : 0;
 	if (fused) *fused = (e->kind == CT_KIND_PLAIN && e->use_fused) ? 1 : 0;
-	if (row_seq) *row_seq = (e->kind == CT_KIND_PLAIN && e->row_seq) ? 1 : 0;
+	if (row_form) *row_form = (e->kind == CT_KIND_PLAIN) ? e->row_form : 0;
 	if (ms) for (int i = 0; i < CT_TUNE_TIMINGS; ++i) ms[i] = e->tune_ms[i];
 	return CT_OK;
 }
@@ -381,7 +390,7 @@ static int retune_for_primes(ct_engine *e)
 		(void)hipGetLastError();      // a failed allocation must not surface at the next launch check
 		return CT_OK;
 	}
-	e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->row_seq = v.row_seq; e->fused_w = 0;
+	e->col_w = v.col_w; e->split_a = v.split; e->use_fused = v.fused; e->row_form = v.row_form; e->fused_w = 0;
 	for (int i = 1; i < CT_TUNE_TIMINGS; ++i) e->tune_ms[i] = v.ms[i];
 	return CT_OK;
 }

@@ -35,7 +35,8 @@ struct ct_engine
 	uint32_t rows_log2 = 0;           // column length R = 2^rows_log2
 	uint32_t col_w = 0;               // column-pass tile width (0 = default for the length)
 	bool split_a = false;             // column pass A: one workgroup per column group and prime
-	bool row_seq = false;             // row pass B: one workgroup per row, its primes in sequence
+	int row_form = 0;                 // row pass B: 0 a workgroup per (row, prime), 1 per row with its primes
+	                                  // in sequence, 2 per row with its primes in workgroup slices
 	bool use_fused = false;           // fuse column pass C with the next pass A
 	uint32_t fused_w = 0;             // tile of the grid that last used the flags (0: clear them)
 	int fused_np = 0;                 // and its prime count
@@ -48,7 +49,7 @@ struct ct_engine
 	int64_t *d_xc = nullptr;          // fused pass: last-chunk carries per (lane, column group, row)
 	// autotuner timings (ms per squaring): resident, multipass W=16, W=32, and
 	// the same two tiles with column pass A split by prime
-	float tune_ms[CT_TUNE_TIMINGS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+	float tune_ms[CT_TUNE_TIMINGS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 	bool tuned = false;               // created with csize 0 (autotuned)
 	int tuned_np = 0;                 // prime count the multipass parameters were tuned for
 	bool pending = false;             // register 0 still owes the chunk carries (plain engine)
@@ -59,7 +60,7 @@ struct ct_engine
 	uint32_t base = 2;                // PRP base: a set exponent bit multiplies by it (2 or 3)
 	// captured squaring sequences of the multipass engine (hipGraph), keyed
 	// by what the captured kernel arguments depend on
-	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool split, row_seq, fused, ticket, first_carry; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
+	struct graph_entry { int nsteps, np; uint32_t col_w, base; bool split, fused, ticket, first_carry; int row_form; const void *bits, *mask, *lane_i; uint32_t words; hipGraphExec_t exec; };
 	std::vector<graph_entry> graphs;
 	bool use_graphs = true;
 	int32_t *d_run = nullptr;         // {top bit index of the run, squarings done by replayed graphs}
@@ -89,6 +90,7 @@ int plain_mul(ct_engine *e, uint32_t dst, uint32_t a, uint32_t b);
 int plain_materialize(ct_engine *e);
 void choose_primes(ct_engine *e);
 bool plain_tile_ok(int ln, uint32_t w);
+int plain_rows_log2(int ln);          // log2 of the column length of the multipass layout
 // The window sieve's primes from 2^32 up: d_list holds kWindow64List words
 // and d_count one word, both owned by the caller.
 constexpr size_t kWindow64List = 2ull << 21;

@@ -45,7 +45,7 @@ struct plain_args
 	int step;                   // squaring index within this launch sequence: bit = top - (iter + step)
 	int base_m1;                // PRP base - 1 (a set bit multiplies by the base)
 	int split_a;                // pass A: one workgroup per column group and prime
-	int row_seq;                // pass B: one workgroup per row, its primes in sequence
+	int row_form;               // pass B: 0 per (row, prime), 1 per row, primes in sequence, 2 per row, primes in slices
 	uint32_t lanes;
 	garner_consts gc;
 };
@@ -302,6 +302,31 @@ __global__ __launch_bounds__((1u << (LN - RL)) / 16) void k_rowB(plain_args A)
 	if (P == 0) rowB_prime<LN, RL, 0, MUL>(A, lds, r, lane, t);
 	else if (P == 1) rowB_prime<LN, RL, 1, MUL>(A, lds, r, lane, t);
 	else rowB_prime<LN, RL, 2, MUL>(A, lds, r, lane, t);
+}
+
+// B with the primes of a row in one workgroup of NP slices, slice P on prime
+// P (every slice whole wavefronts, so the prime is uniform over a wavefront
+// and every slice passes the same barriers); each slice exchanges through its
+// own LDS region.  One workgroup per row instead of NP.
+template <int LN, int RL, int NP>
+struct rowh
+{
+	static constexpr uint32_t T = (1u << (LN - RL)) / 16;
+	static constexpr bool ok = (T % 64 == 0) && (NP * T <= 1024);
+};
+
+template <int LN, int RL, int NP>
+__global__ __launch_bounds__((NP * rowh<LN, RL, NP>::T)) void k_rowH(plain_args A)
+{
+	constexpr int CL = LN - RL;
+	constexpr uint32_t T = rowh<LN, RL, NP>::T;
+	static_assert(rowh<LN, RL, NP>::ok, "slices of whole wavefronts, at most 1024 threads");
+	__shared__ uint32_t lds[NP][geo<CL>::lds_words];
+	const uint32_t r = blockIdx.x, lane = blockIdx.y, P = threadIdx.x / T, t = threadIdx.x % T;
+	if (A.lane_i && A.lane_i[lane] < 0) return;
+	if (P == 0) rowB_prime<LN, RL, 0, false>(A, lds[0], r, lane, t);
+	else if (P == 1) rowB_prime<LN, RL, 1, false>(A, lds[1], r, lane, t);
+	else rowB_prime<LN, RL, 2, false>(A, lds[NP - 1], r, lane, t);
 }
 
 // B with the primes of a row in one workgroup, one after the other: the next
@@ -869,8 +894,19 @@ static hipError_t square_ln(const plain_args &A, uint32_t lanes, hipStream_t st,
 	}
 	if (parts & 2)
 	{
-		if (A.row_seq) hipLaunchKernelGGL((k_rowS<LN, RL, NP>), dim3(R, lanes), dim3(C / 16), 0, st, A);
-		else hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
+		// row form 2 where its slices fit one workgroup, else form 0
+		bool done = false;
+		if constexpr (rowh<LN, RL, NP>::ok)
+			if (A.row_form == 2)
+			{
+				hipLaunchKernelGGL((k_rowH<LN, RL, NP>), dim3(R, lanes), dim3(NP * (C / 16)), 0, st, A);
+				done = true;
+			}
+		if (!done)
+		{
+			if (A.row_form == 1) hipLaunchKernelGGL((k_rowS<LN, RL, NP>), dim3(R, lanes), dim3(C / 16), 0, st, A);
+			else hipLaunchKernelGGL((k_rowB<LN, RL, false>), dim3(R, NP, lanes), dim3(C / 16), 0, st, A);
+		}
 	}
 	if (parts & 4) hipLaunchKernelGGL((k_colC<LN, RL, NP, W, MINUS>), gcol, dim3(G::threads), 0, st, A, 0);
 	return hipGetLastError();
@@ -1021,6 +1057,7 @@ static hipError_t mul_any(int tln, int np, uint32_t w, bool minus, const plain_a
 #define HIPCHK(call) do { hipError_t _e = (call); if (_e != hipSuccess) return hip_fail(_e, #call); } while (0)
 
 bool plain_tile_ok(int ln, uint32_t w) { return tile_ok(ln, w); }
+int plain_rows_log2(int ln) { return rows_log2(ln); }
 
 int plain_alloc(ct_engine *e)
 {
@@ -1073,7 +1110,7 @@ static plain_args make_args(ct_engine *e)
 	A.res = e->d_res; A.res2 = e->d_res2; A.carry = e->d_carry;
 	A.tw = e->d_tw; A.dv = e->d_dv; A.lanes = e->lanes; A.gc = e->gc; A.run = e->d_run; A.base_m1 = (int)e->base - 1;
 	A.split_a = e->split_a ? 1 : 0;
-	A.row_seq = e->row_seq ? 1 : 0;
+	A.row_form = e->row_form;
 	return A;
 }
 
@@ -1152,7 +1189,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, bool fused, hipGraphEx
 	const bool fc = e->pending;
 	for (const auto &g : e->graphs)
 		if (g.nsteps == len && g.np == e->np && g.col_w == e->col_w && g.base == e->base && g.split == e->split_a &&
-		    g.row_seq == e->row_seq &&
+		    g.row_form == e->row_form &&
 		    g.fused == fused && g.ticket == (fused && e->fused_ticket) && g.first_carry == fc && g.bits == A.bits &&
 		    g.mask == A.mask &&
 		    g.lane_i == A.lane_i && g.words == A.bit_words) { *out = g.exec; return CT_OK; }
@@ -1175,7 +1212,7 @@ static int graph_for(ct_engine *e, plain_args A, int len, bool fused, hipGraphEx
 	h = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	hipGraphDestroy(graph);
 	if (h != hipSuccess) return hip_fail(h, "hipGraphInstantiate");
-	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, e->row_seq, fused, fused && e->fused_ticket, fc, A.bits,
+	e->graphs.push_back({len, e->np, e->col_w, e->base, e->split_a, fused, fused && e->fused_ticket, fc, e->row_form, A.bits,
 	                     A.mask, A.lane_i, A.bit_words, exec});
 	*out = exec;
 	return CT_OK;

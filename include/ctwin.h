@@ -89,21 +89,23 @@ int ct_engine_set_column_split(ct_engine *e, int on);
  * reference. */
 int ct_engine_set_fused(ct_engine *e, int on);
 
-/* Multipass engine: row pass B as one workgroup per row and prime (seq = 0)
- * or one workgroup per row that transforms its primes one after the other,
- * loading the next prime's row while it transforms the current one (seq = 1:
- * half the workgroups).  Results do not depend on it.  No counterpart in the
- * reference. */
-int ct_engine_set_row_form(ct_engine *e, int seq);
+/* Multipass engine: the workgroup form of row pass B.  form 0: one workgroup
+ * per row and prime; 1: one workgroup per row that transforms its primes one
+ * after the other, loading the next prime's row while it transforms the
+ * current one; 2: one workgroup per row with one slice of threads per prime
+ * (used where the slices are whole wavefronts and fit 1024 threads, else
+ * form 0).  Results do not depend on it.  No counterpart in the reference. */
+int ct_engine_set_row_form(ct_engine *e, int form);
 
 /* What the autotuner chose and measured: the column tile (0 for resident
  * kernels), the split of the column passes, the fused column pass, the row
- * form, and ms per squaring (-1 where not timed or when the engine was not
- * tuned) in ms[CT_TUNE_TIMINGS]: {resident, multipass W=16, W=32, W=16 split,
- * W=32 split, W=16 fused, W=32 fused, W=16 fused split, W=32 fused split, the
- * fastest of these with the sequential row form}.  Any output may be null. */
-#define CT_TUNE_TIMINGS 10
-int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, int *row_seq, float *ms);
+ * form (0, 1 or 2, ct_engine_set_row_form), and ms per squaring (-1 where not
+ * timed or when the engine was not tuned) in ms[CT_TUNE_TIMINGS]: {resident,
+ * multipass W=16, W=32, W=16 split, W=32 split, W=16 fused, W=32 fused, W=16
+ * fused split, W=32 fused split, the fastest of these with row form 1, with
+ * row form 2}.  Any output may be null. */
+#define CT_TUNE_TIMINGS 11
+int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, int *row_form, float *ms);
 int ct_engine_destroy(ct_engine *e);
 
 /* transform::set_b (transform.h:472-486): b[lanes], 32 < b < 2^31. */

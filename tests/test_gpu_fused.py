@@ -25,7 +25,8 @@ def _exponents(bs, n):
     (13, 1, 16, True, 1, False, False), (13, 1, 16, False, 1, True, True), (16, 2, 16, True, 1, False, True),
     (16, 3, 32, False, 1, True, False), (16, 3, 16, True, 2, False, True), (21, 1, 16, True, 1, False, False),
     (21, 1, 32, True, 1, False, False), (21, 1, 32, False, 1, False, True), (21, 1, 32, True, 2, False, True),
-    (21, 1, 32, True, 1, True, True)])
+    (21, 1, 32, True, 1, True, True), (18, 2, 16, True, 1, 2, True), (21, 1, 32, True, 1, 2, True),
+    (21, 1, 32, False, 1, 2, False)])
 def test_fused_equals_three_pass(ln, lanes, tile, graphs, mode, seq, split):
     n = 1 << ln
     rng = random.Random(1500 + ln + tile)
@@ -39,7 +40,7 @@ def test_fused_equals_three_pass(ln, lanes, tile, graphs, mode, seq, split):
         t.set_column_tile(tile)
         t.set_fused(fused)
         t.set_graphs(graphs)
-        t.set_row_form(seq and fused)           # the reference engine keeps one workgroup per row and prime
+        t.set_row_form(int(seq) if fused else 0)   # the reference engine keeps one workgroup per row and prime
         t.set_column_split(split and fused)     # and one prime per thread
         t.set_b(bs); t.set_exponents(E); t.set(1); t.copy(1, 0)
         engines.append(t)

@@ -44,7 +44,7 @@ def test_tuned_engine_matches_untuned(ln, lanes):
             idx = (5 if w == 16 else 6) + (2 if split else 0)
         else:
             idx = (1 if w in (16, 64) else 2) + (2 if split else 0)
-        assert ms[9 if rs else idx] == best
+        assert ms[8 + rs if rs else idx] == best
     ref = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=16)
     a = _run(tuned, bs, n, 40, 1)
     b = _run(ref, bs, n, 40, 1)
@@ -55,7 +55,7 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     if tuned.kind == ct.KIND_PLAIN:
         timed2 = [m for m in ms2[1:] if m > 0]
         idx2 = ((5 if w2 == 16 else 6) if fused2 else (1 if w2 in (16, 64) else 2)) + (2 if split2 else 0)
-        assert ms2[9 if rs2 else idx2] == min(timed2)
+        assert ms2[8 + rs2 if rs2 else idx2] == min(timed2)
     # second creation hits the cache: same choice, same timings
     again = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
     assert again.kind == tuned.kind and again.tuning() == (w, split, fused, ms, rs)
@@ -70,12 +70,12 @@ def test_column_tiles_give_identical_digits(ln):
     out = []
     for w in (16, 32):
         for split in (False, True):
-            for seq in (False, True):
+            for form in (0, 1, 2):
                 t = ct.Transform(ln, 1, ct.KIND_PLAIN)
                 t.set_column_tile(w)
                 t.set_column_split(split)
-                t.set_row_form(seq)
-                assert t.tuning()[:3] == (w, split, False) and t.tuning()[4] == seq
+                t.set_row_form(form)
+                assert t.tuning()[:3] == (w, split, False) and t.tuning()[4] == form
                 out.append(_run(t, bs, n, 30, 2))
     for o in out[1:]:
         assert np.array_equal(out[0], o)
