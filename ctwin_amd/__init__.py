@@ -38,7 +38,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free",
            "ct_sieve_factors_noalloc", "ct_engine_set_row_form", "ct_set_graphs_default",
            "ct_fsieve_create", "ct_fsieve_blocks", "ct_fsieve_set_bmax", "ct_fsieve_run", "ct_fsieve_destroy",
-           "ct_fsieve_check", "ct_fsieve_quit", "ct_sieve_last_timing"]
+           "ct_fsieve_check", "ct_fsieve_quit", "ct_sieve_last_timing", "ct_merge_factors"]
 
 INTERRUPTED = 1
 
@@ -51,6 +51,12 @@ class CycloStats(ctypes.Structure):
 class FsieveStats(ctypes.Structure):
     _fields_ = [("blocks_done", ctypes.c_uint64), ("blocks_total", ctypes.c_uint64), ("factors", ctypes.c_uint64),
                 ("elapsed", ctypes.c_double), ("resumed", ctypes.c_int), ("interrupted", ctypes.c_int)]
+
+
+class MergeStats(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("in_window", ctypes.c_uint64),
+                ("factors", ctypes.c_uint64), ("bad", ctypes.c_uint64), ("out_of_range", ctypes.c_uint64),
+                ("remaining", ctypes.c_uint64)]
 
 
 class SieveTiming(ctypes.Structure):
@@ -132,6 +138,8 @@ def lib():
             "ct_fsieve_run": [vp, u64, u64, u64p, u64, c.POINTER(u64)],
             "ct_fsieve_destroy": [vp],
             "ct_fsieve_check": [i32, i32, u64, u64, i32, c.c_char_p, c.c_double, u64, c.POINTER(FsieveStats)],
+            "ct_merge_factors": [i32, i32, u64, u64, c.c_char_p, i32, c.POINTER(c.c_char_p), c.POINTER(i32),
+                                 c.POINTER(u64), c.POINTER(u64), c.c_char_p, c.POINTER(MergeStats)],
             "ct_sieve_last_timing": [i32, c.POINTER(SieveTiming)],
             "ct_engine_set_column_split": [vp, i32],
             "ct_engine_set_fused": [vp, i32],
@@ -545,6 +553,24 @@ def fsieve_check(m, p_min, p_max, mode, work_dir, record_s=120.0, max_blocks=0, 
 
 def fsieve_quit():
     lib().ct_fsieve_quit()
+
+
+def merge_factors(m, b_min, b_max, cand_path, factor_files, out_path, device=0):
+    """The ctools merge: remove from the candidate list every b with a
+    verified factor.  factor_files: [(path, mode, p_min_t, p_max_t)] with
+    mode +1 (fp file) or -1 (fn file) and the file's p range in units of
+    10^12 (None: no range check).  Returns the stats dict."""
+    k = len(factor_files)
+    paths = (ctypes.c_char_p * max(k, 1))(*[os.fsencode(f[0]) for f in factor_files])
+    modes = (ctypes.c_int * max(k, 1))(*[int(f[1]) for f in factor_files])
+    ranged = k > 0 and all(f[2] is not None and f[3] is not None for f in factor_files)
+    lo = (ctypes.c_uint64 * max(k, 1))(*[int(f[2]) if ranged else 0 for f in factor_files])
+    hi = (ctypes.c_uint64 * max(k, 1))(*[int(f[3]) if ranged else 0 for f in factor_files])
+    st = MergeStats()
+    _check(lib().ct_merge_factors(device, m, int(b_min), int(b_max), os.fsencode(cand_path), k, paths, modes,
+                                  lo if ranged else None, hi if ranged else None, os.fsencode(out_path),
+                                  ctypes.byref(st)))
+    return {f: getattr(st, f) for f, _ in MergeStats._fields_}
 
 
 def crc32(data, crc=0):

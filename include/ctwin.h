@@ -355,6 +355,32 @@ int ct_fsieve_check(int device, int m, uint64_t p_min, uint64_t p_max, int mode,
  * the reference's quit flag, ctsieve.h:166-170). */
 void ct_fsieve_quit(void);
 
+/* The ctools merge (tools/ctools.cpp:151-248, read_cand / read_factors /
+ * write_cand): removes from a candidate list every b with a known factor.
+ * - cand_path: one b per line; a b not above the previous one is skipped.
+ * - factor_paths[i] (mode modes[i]: +1 for fp files, p | N+; -1 for fn
+ *   files, p | N-): "p b" lines; a line repeating the previous pair is
+ *   skipped.  A pair with b_min <= b <= b_max is verified on the device
+ *   (x = b^(2^(m-1)) mod p, x^2 - x = -1 for +, +1 for -, p < 2^64) and, if
+ *   it holds, removes b.  p outside [p_min_t[i], p_max_t[i]] x 10^12 is counted
+ *   as out of range and still used; p_min_t / p_max_t may be NULL.
+ * - out_path: the remaining candidates, one per line, ascending.
+ * stats may be NULL.  A pair that fails the check is counted, not fatal, as
+ * in the reference. */
+typedef struct
+{
+	uint64_t candidates;    /* candidates read */
+	uint64_t pairs;         /* pairs read, repeated lines dropped */
+	uint64_t in_window;     /* pairs with b_min <= b <= b_max */
+	uint64_t factors;       /* pairs that hold */
+	uint64_t bad;           /* pairs in the window that do not hold */
+	uint64_t out_of_range;  /* pairs with p outside their file's p range */
+	uint64_t remaining;     /* candidates written */
+} ct_merge_stats;
+int ct_merge_factors(int device, int m, uint64_t b_min, uint64_t b_max, const char *cand_path, int nfiles,
+                     const char *const *factor_paths, const int *modes, const uint64_t *p_min_t,
+                     const uint64_t *p_max_t, const char *out_path, ct_merge_stats *stats);
+
 /* ---------------- files and the batch driver ---------------- */
 
 /* CRC-32 (IEEE 802.3, reflected) continued over len more bytes; the files
