@@ -37,38 +37,53 @@ __global__ void k_merge_verify(const uint64_t *p, const uint64_t *b, uint64_t co
 
 namespace {
 
-// whole file into memory (the lists are read once, front to back)
-bool slurp(const char *path, std::string &out)
+// Unsigned decimal integers of a text file, read front to back through a
+// fixed buffer (factor files can be larger than memory): next() skips any
+// non-digit bytes and returns the following number; a number above 2^64 - 1
+// sets overflow.
+class number_reader
 {
-	FILE *f = std::fopen(path, "rb");
-	if (!f) return false;
-	std::fseek(f, 0, SEEK_END);
-	const long sz = std::ftell(f);
-	std::fseek(f, 0, SEEK_SET);
-	out.resize(sz > 0 ? (size_t)sz : 0);
-	const size_t got = out.empty() ? 0 : std::fread(&out[0], 1, out.size(), f);
-	std::fclose(f);
-	return got == out.size();
-}
-
-// next unsigned decimal integer at or after pos (false at the end); values
-// above 2^64 - 1 are rejected
-bool next_u64(const std::string &s, size_t &pos, uint64_t &v, bool &overflow)
-{
-	const size_t n = s.size();
-	while (pos < n && (s[pos] < '0' || s[pos] > '9')) ++pos;
-	if (pos >= n) return false;
-	v = 0;
-	overflow = false;
-	while (pos < n && s[pos] >= '0' && s[pos] <= '9')
+public:
+	explicit number_reader(const char *path) : f_(std::fopen(path, "rb")), buf_(1u << 24) {}
+	~number_reader() { if (f_) std::fclose(f_); }
+	number_reader(const number_reader &) = delete;
+	number_reader &operator=(const number_reader &) = delete;
+	bool ok() const { return f_ != nullptr; }
+	bool error() const { return err_; }
+	bool next(uint64_t &v, bool &overflow)
 	{
-		const uint64_t d = (uint64_t)(s[pos] - '0');
-		if (v > (~0ull - d) / 10) overflow = true;
-		v = v * 10 + d;
-		++pos;
+		int c;
+		while ((c = get()) >= 0 && (c < '0' || c > '9')) {}
+		if (c < 0) return false;
+		v = 0;
+		overflow = false;
+		for (; c >= '0' && c <= '9'; c = get())
+		{
+			const uint64_t d = (uint64_t)(c - '0');
+			if (v > (~0ull - d) / 10) overflow = true;
+			v = v * 10 + d;
+		}
+		return true;
 	}
-	return true;
-}
+
+private:
+	int get()
+	{
+		if (pos_ == len_)
+		{
+			if (!f_ || eof_) return -1;
+			len_ = std::fread(buf_.data(), 1, buf_.size(), f_);
+			pos_ = 0;
+			if (len_ < buf_.size()) { eof_ = true; err_ = std::ferror(f_) != 0; }
+			if (len_ == 0) return -1;
+		}
+		return (unsigned char)buf_[pos_++];
+	}
+	FILE *f_;
+	std::vector<char> buf_;
+	size_t pos_ = 0, len_ = 0;
+	bool eof_ = false, err_ = false;
+};
 
 constexpr uint64_t kBatch = 1ull << 22;       // pairs per device batch
 constexpr uint64_t kTera = 1000000000000ull;
@@ -94,17 +109,17 @@ int ct_merge_factors(int device, int m, uint64_t b_min, uint64_t b_max, const ch
 	// read_cand (ctools.cpp:151-175): a b not above the previous one is skipped
 	std::vector<uint64_t> cand;
 	{
-		std::string text;
-		if (!slurp(cand_path, text)) return fail(CT_ERR_ARG, "cannot read the candidate file");
-		size_t pos = 0;
+		number_reader in(cand_path);
+		if (!in.ok()) return fail(CT_ERR_ARG, "cannot read the candidate file");
 		uint64_t v = 0, prev = 0;
 		bool ovf = false;
-		while (next_u64(text, pos, v, ovf))
+		while (in.next(v, ovf))
 		{
 			if (ovf) return fail(CT_ERR_ARG, "candidate above 2^64 - 1");
 			if (v > prev) cand.push_back(v);
 			prev = v;
 		}
+		if (in.error()) return fail(CT_ERR_ARG, "read error in the candidate file");
 	}
 	st.candidates = cand.size();
 	std::vector<uint8_t> removed(cand.size(), 0);
@@ -115,12 +130,22 @@ int ct_merge_factors(int device, int m, uint64_t b_min, uint64_t b_max, const ch
 	hipStream_t hs = nullptr;
 	std::vector<uint64_t> bp, bb;
 	std::vector<uint8_t> ok;
-	bp.reserve(kBatch); bb.reserve(kBatch);
+	bp.reserve(1u << 16); bb.reserve(1u << 16);
 	// device verification of the pending batch, then the removals
 	auto flush = [&](int mode) -> int {
 		if (bp.empty()) return CT_OK;
 		const uint64_t cnt = bp.size();
-		hipError_t h = hipMemcpyAsync(d_p, bp.data(), cnt * sizeof(uint64_t), hipMemcpyHostToDevice, hs);
+		hipError_t h = hipSuccess;
+		if (!hs)
+		{   // the device is touched only when some pair lies in the window
+			h = hipSetDevice(device);
+			if (h == hipSuccess) h = hipStreamCreate(&hs);
+			if (h == hipSuccess) h = hipMalloc((void **)&d_p, kBatch * sizeof(uint64_t));
+			if (h == hipSuccess) h = hipMalloc((void **)&d_b, kBatch * sizeof(uint64_t));
+			if (h == hipSuccess) h = hipMalloc((void **)&d_ok, kBatch);
+			if (h != hipSuccess) return hip_fail(h, "merge: device buffers");
+		}
+		h = hipMemcpyAsync(d_p, bp.data(), cnt * sizeof(uint64_t), hipMemcpyHostToDevice, hs);
 		if (h == hipSuccess) h = hipMemcpyAsync(d_b, bb.data(), cnt * sizeof(uint64_t), hipMemcpyHostToDevice, hs);
 		if (h != hipSuccess) return hip_fail(h, "merge: copy of the pairs");
 		const uint32_t grid = (uint32_t)std::min<uint64_t>((cnt + 255) / 256, 4096);
@@ -141,30 +166,21 @@ int ct_merge_factors(int device, int m, uint64_t b_min, uint64_t b_max, const ch
 		bp.clear(); bb.clear();
 		return CT_OK;
 	};
-	{
-		hipError_t h = hipSetDevice(device);
-		if (h == hipSuccess) h = hipStreamCreate(&hs);
-		if (h == hipSuccess) h = hipMalloc((void **)&d_p, kBatch * sizeof(uint64_t));
-		if (h == hipSuccess) h = hipMalloc((void **)&d_b, kBatch * sizeof(uint64_t));
-		if (h == hipSuccess) h = hipMalloc((void **)&d_ok, kBatch);
-		if (h != hipSuccess) { rc = hip_fail(h, "merge: device buffers"); goto done; }
-	}
 	// read_factors (ctools.cpp:177-224), one file after the other
 	for (int f = 0; f < nfiles && rc == CT_OK; ++f)
 	{
-		std::string text;
-		if (!slurp(factor_paths[f], text)) { rc = fail(CT_ERR_ARG, "cannot read a factor file"); break; }
+		number_reader in(factor_paths[f]);
+		if (!in.ok()) { rc = fail(CT_ERR_ARG, "cannot read a factor file"); break; }
 		const bool ranged = p_min_t && p_max_t;
 		const unsigned __int128 lo = ranged ? (unsigned __int128)p_min_t[f] * kTera : 0;
 		const unsigned __int128 hi = ranged ? (unsigned __int128)p_max_t[f] * kTera : 0;
-		size_t pos = 0;
 		uint64_t p = 0, b = 0, p_prev = 0, b_prev = 0;
 		bool ovf = false;
 		while (true)
 		{
-			if (!next_u64(text, pos, p, ovf)) break;
+			if (!in.next(p, ovf)) break;
 			if (ovf) { rc = fail(CT_ERR_ARG, "factor above 2^64 - 1"); break; }
-			if (!next_u64(text, pos, b, ovf)) { rc = fail(CT_ERR_ARG, "factor file line without b"); break; }
+			if (!in.next(b, ovf)) { rc = fail(CT_ERR_ARG, "factor file line without b"); break; }
 			if (ovf) { rc = fail(CT_ERR_ARG, "b above 2^64 - 1"); break; }
 			if (p == p_prev && b == b_prev) continue;
 			p_prev = p; b_prev = b;
@@ -175,6 +191,7 @@ int ct_merge_factors(int device, int m, uint64_t b_min, uint64_t b_max, const ch
 			bp.push_back(p); bb.push_back(b);
 			if (bp.size() == kBatch && (rc = flush(modes[f])) != CT_OK) break;
 		}
+		if (rc == CT_OK && in.error()) rc = fail(CT_ERR_ARG, "read error in a factor file");
 		if (rc == CT_OK) rc = flush(modes[f]);
 	}
 	// write_cand (ctools.cpp:226-248)
@@ -185,17 +202,21 @@ int ct_merge_factors(int device, int m, uint64_t b_min, uint64_t b_max, const ch
 		std::string buf;
 		buf.reserve(1 << 20);
 		char tmp[24];
+		bool wrote = true;
 		for (size_t i = 0; i < cand.size(); ++i)
 		{
 			if (removed[i]) continue;
 			++st.remaining;
 			const int len = std::snprintf(tmp, sizeof(tmp), "%llu\n", (unsigned long long)cand[i]);
 			buf.append(tmp, (size_t)len);
-			if (buf.size() >= (1u << 20)) { std::fwrite(buf.data(), 1, buf.size(), out); buf.clear(); }
+			if (buf.size() >= (1u << 20))
+			{
+				wrote = wrote && std::fwrite(buf.data(), 1, buf.size(), out) == buf.size();
+				buf.clear();
+			}
 		}
-		const size_t w = std::fwrite(buf.data(), 1, buf.size(), out);
-		const int cl = std::fclose(out);
-		if (w != buf.size() || cl != 0) rc = fail(CT_ERR_ARG, "short write of the candidate file");
+		wrote = wrote && std::fwrite(buf.data(), 1, buf.size(), out) == buf.size();
+		if (std::fclose(out) != 0 || !wrote) rc = fail(CT_ERR_ARG, "short write of the candidate file");
 	}
 done:
 	if (d_p) (void)hipFree(d_p);
