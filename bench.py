@@ -177,11 +177,34 @@ def prp_work(i0):
     return (i0 + 1) + 2 * L + (B + 11) + 4
 
 
-def run_plain(ct, ln, b, steps, warmup, graphs=True, config=None):
+LARGE_FIXTURE = os.path.join(ROOT, "tests", "golden", "large_n2e21_b73764.jsonl")
+
+
+def large_parity(ct, t, b, ln, done):
+    """After `done` squarings from x = 1 at the exponent's top bit: RES64 and
+    x mod three 31-bit primes of the canonical residue against GMP's
+    checkpoint at t = done (tests/golden/large_n2e21_b73764.jsonl)."""
+    if not os.path.exists(LARGE_FIXTURE):
+        return {"checked": False, "reason": "fixture missing"}
+    lines = [json.loads(l) for l in open(LARGE_FIXTURE) if l.strip()]
+    head, checks = lines[0], {c["t"]: c for c in lines[1:]}
+    if head["b"] != b or head["log2_n"] != ln or done not in checks:
+        return {"checked": False, "reason": f"no GMP checkpoint for b={b}, n=2^{ln}, t={done}"}
+    r64, mods = ct.residue_checks(t.get_int(0), [b], head["q"])
+    want = checks[done]
+    ok = "%016X" % r64[0] == want["res64"] and mods[0] == want["mod_q"]
+    return {"checked": True, "squarings": done, "bit_exact": ok,
+            "against": "GMP (tests/golden/large_n2e21_b73764.jsonl): RES64 and x mod 3 primes"}
+
+
+def run_plain(ct, ln, b, steps, warmup, graphs=True, config=None, check=False):
     """Plain multipass engine, one candidate: `warmup` untimed squarings, then
     `steps` timed ones.  The warm-up also runs one run of the timed length, so
     that the captured graph of that length exists (a long PRP test replays it
-    millions of times).  config = (tile, split, fused) instead of the autotuner."""
+    millions of times).  config = (tile, split, fused) instead of the autotuner.
+    check: the timed run restarts from x = 1 at the exponent's top bit (a
+    squaring costs the same whatever the digits) and its result is compared
+    with GMP (large_parity)."""
     from ctwin_amd.exponent import window_value
     n = 1 << ln
     Ew, i0 = window_value(b, n, 2 * steps + warmup + 8)
@@ -195,14 +218,19 @@ def run_plain(ct, ln, b, steps, warmup, graphs=True, config=None):
     t.run(i0, i0 - warmup + 1)
     t.run(i0 - warmup, i0 - warmup - steps + 1)
     i = i0 - warmup - steps
+    if check:
+        t.set(1)
+        i = i0
     t.timer_start()
     t.run(i, i - steps + 1)
     ms = t.timer_stop()
     w, split, fused, _, row_form = t.tuning()
     t.read_state(0)                 # the read checks the fused pass's carry exchanges
+    cfg = {"column_tile": w, "split_pass_a": split, "fused_pass_ca": fused, "row_form": row_form}
+    if check:
+        cfg["parity"] = large_parity(ct, t, b, ln, steps)
     t.close()
-    return steps / (ms / 1e3), ms, {"column_tile": w, "split_pass_a": split, "fused_pass_ca": fused,
-                                    "row_form": row_form}
+    return steps / (ms / 1e3), ms, cfg
 
 
 def run_sieve_config2(ct):
@@ -478,8 +506,9 @@ def main():
     plain_sq_s, plain_ms, plain_cfg = run_plain(ct, 14, bs[0], args.plain_steps, 5, graphs=not args.no_graphs)
     print(f"[bench] plain n=2^14 x1: {plain_sq_s:.4g} squarings/s", flush=True)
     big_b = 65536 + 2 * random.Random(SEED).randrange(1 << 14)
-    big_sq_s, big_ms, big_cfg = run_plain(ct, 21, big_b, args.big_steps, 5, graphs=not args.no_graphs)
-    print(f"[bench] plain n=2^21 x1: {big_sq_s:.4g} squarings/s", flush=True)
+    big_sq_s, big_ms, big_cfg = run_plain(ct, 21, big_b, args.big_steps, 5, graphs=not args.no_graphs, check=True)
+    big_parity = big_cfg.pop("parity")
+    print(f"[bench] plain n=2^21 x1: {big_sq_s:.4g} squarings/s; parity {big_parity}", flush=True)
 
     sieve = run_sieve_config2(ct)
     print(f"[bench] sieve config 2: {sieve['survivors']} survivors in {sieve['seconds']:.3f} s", flush=True)
@@ -525,7 +554,7 @@ def main():
                   "engine": "plain multipass, one candidate per launch sequence", "tuned": plain_cfg},
         "batched_vs_plain": sq_s / plain_sq_s,
         "single_large": {"log2_n": 21, "b": big_b, "squarings_per_s": big_sq_s, "ms_per_squaring": big_ms / args.big_steps,
-                         "tuned": big_cfg,
+                         "tuned": big_cfg, "parity": big_parity,
                          "roofline": {"bound": "hbm", "achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": hbm_achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_N2E21_SQUARING,
                                       "bytes_alg_per_squaring": bytes_alg(big_n),
