@@ -29,7 +29,8 @@ SEED = 20261203
 SQ_PER_STEP = 250           # squarings of every batched candidate per bench step
 TUNE_NAMES = ("resident", "multipass_w16", "multipass_w32", "multipass_w16_split", "multipass_w32_split",
               "multipass_w16_fused", "multipass_w32_fused", "multipass_w16_fused_split", "multipass_w32_fused_split",
-              "multipass_best_row_primes_in_sequence", "multipass_best_row_primes_in_slices")
+              "multipass_best_row_primes_in_sequence", "multipass_best_row_primes_in_slices",
+              "resident_two_primes", "resident_two_primes_in_halves")
 
 
 def mul32_alg(n, primes=3):

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libctwin_amd.so")
 
 KIND_AUTO, KIND_RESIDENT, KIND_PLAIN = 0, 1, 2
-TUNE_TIMINGS = 11                    # CT_TUNE_TIMINGS
+TUNE_TIMINGS = 13                    # CT_TUNE_TIMINGS
 
 STATUS = {0: "CT_OK", -1: "CT_ERR_ARG", -2: "CT_ERR_HIP", -3: "CT_ERR_NOMEM", -4: "CT_ERR_UNSUPPORTED",
           -5: "CT_ERR_GETINT", -6: "CT_ERR_COMPUTE", -7: "CT_ERR_GERBICZ"}
@@ -36,7 +36,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
            "ct_engine_create_ex", "ct_engine_set_base", "ct_engine_set_column_split",
            "ct_engine_set_fused", "ct_prp_begin", "ct_prp_finish", "ct_prp_job_free",
-           "ct_sieve_factors_noalloc", "ct_engine_set_row_form", "ct_set_graphs_default",
+           "ct_sieve_factors_noalloc", "ct_engine_set_row_form", "ct_engine_set_prime_halves", "ct_set_graphs_default",
            "ct_fsieve_create", "ct_fsieve_blocks", "ct_fsieve_set_bmax", "ct_fsieve_run", "ct_fsieve_destroy",
            "ct_fsieve_check", "ct_fsieve_quit", "ct_sieve_last_timing", "ct_merge_factors"]
 
@@ -131,6 +131,7 @@ def lib():
             "ct_engine_tuning": [vp, c.POINTER(u32), c.POINTER(i32), c.POINTER(i32), c.POINTER(i32),
                                  np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")],
             "ct_engine_set_row_form": [vp, i32],
+            "ct_engine_set_prime_halves": [vp, i32],
             "ct_set_graphs_default": [i32],
             "ct_fsieve_create": [i32, i32, u64, u64, i32, c.POINTER(vp)],
             "ct_fsieve_blocks": [vp, c.POINTER(u64), c.POINTER(u64)],
@@ -267,9 +268,11 @@ class Transform:
         """(column tile, split of column pass A, fused column pass, ms per
         squaring for {resident, multipass W=16, W=32, W=16 split, W=32 split,
         W=16 fused, W=32 fused, W=16 fused split, W=32 fused split, the
-        fastest with row form 1, with row form 2}; -1 = not timed, row form
-        0/1/2).  The row form is last so that the first four entries keep
-        their meaning."""
+        fastest with row form 1, with row form 2, resident with two primes one
+        after the other, and in workgroup halves}; -1 = not timed, row form
+        0/1/2).  For a resident engine the split flag reports the primes in
+        workgroup halves.  The row form is last so that the first four
+        entries keep their meaning."""
         w, sp, fu, rf = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = np.zeros(TUNE_TIMINGS, np.float32)
         _check(lib().ct_engine_tuning(self.h, ctypes.byref(w), ctypes.byref(sp), ctypes.byref(fu), ctypes.byref(rf), ms))
@@ -281,6 +284,11 @@ class Transform:
         workgroup per row with a slice of threads per prime (2; used where
         the slices are whole wavefronts within 1024 threads, else 0)."""
         _check(lib().ct_engine_set_row_form(self.h, int(form)))
+
+    def set_prime_halves(self, on):
+        """Resident engine: the two primes of a two-prime squaring in the
+        halves of one workgroup (transform lengths 2^10 .. 2^13)."""
+        _check(lib().ct_engine_set_prime_halves(self.h, int(bool(on))))
 
     def set_fused(self, on):
         """Multipass engine (N+, n >= 2^13): column pass C fused with the next

@@ -120,7 +120,7 @@ namespace {
 struct tune_key { int device; uint32_t ln, lanes; int kind, sign, np;
 	bool operator<(const tune_key &o) const
 	{ return std::tie(device, ln, lanes, kind, sign, np) < std::tie(o.device, o.ln, o.lanes, o.kind, o.sign, o.np); } };
-struct tune_val { int kind; uint32_t col_w; bool split, fused; int row_form; float ms[CT_TUNE_TIMINGS]; };
+struct tune_val { int kind; uint32_t col_w; bool split, fused; int row_form; bool res_halves; float ms[CT_TUNE_TIMINGS]; };
 std::mutex g_tune_mu;
 std::map<tune_key, tune_val> g_tune;
 
@@ -160,7 +160,7 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, int
 		auto it = g_tune.find(key);
 		if (it != g_tune.end()) { *out = it->second; return CT_OK; }
 	}
-	tune_val v{CT_KIND_AUTO, 0, false, false, 0, {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+	tune_val v{CT_KIND_AUTO, 0, false, false, 0, false, {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
 	float best = 1e30f;
 	const bool try_res = (kind == CT_KIND_RESIDENT) || (kind == CT_KIND_AUTO && tln <= 14);
 	const bool try_plain = (kind == CT_KIND_PLAIN) || (kind == CT_KIND_AUTO && tln >= (sign < 0 ? 11u : 10u));
@@ -170,6 +170,20 @@ static int tune(int device, uint32_t ln, uint32_t lanes, int kind, int sign, int
 		int rc = create_impl(device, ln, lanes, CT_KIND_RESIDENT, 0, false, sign, &e);
 		if (rc) return rc;
 		v.ms[0] = time_squarings(e, np);
+		if (v.ms[0] >= 0 && resident_halves_ok((int)tln))
+		{   // two primes (the configuration whose primes can run in workgroup
+			// halves): one prime after the other, and in halves
+			v.ms[11] = time_squarings(e, 2);
+			e->res_halves = true;
+			v.ms[12] = time_squarings(e, 2);
+			e->res_halves = false;
+			if (v.ms[11] >= 0 && v.ms[12] >= 0) v.res_halves = v.ms[12] < v.ms[11];
+			else
+			{   // not timed (the size takes three primes): the one-after-the-other form
+				(void)hipGetLastError();
+				v.ms[11] = v.ms[12] = -1;
+			}
+		}
 		ct_engine_destroy(e);
 		if (v.ms[0] < 0) return fail(CT_ERR_HIP, "autotuner: timing the resident kernels failed");
 		if (v.ms[0] < best) { best = v.ms[0]; v.kind = CT_KIND_RESIDENT; v.col_w = 0; }
@@ -266,6 +280,7 @@ int ct_engine_create_ex(int device, uint32_t log2_n, uint32_t lanes, uint32_t cs
 		std::memcpy((*out)->tune_ms, v.ms, sizeof(v.ms));
 		(*out)->use_fused = v.fused;
 		(*out)->row_form = v.row_form;
+		(*out)->res_halves = v.res_halves;
 		(*out)->tuned = true;
 		(*out)->tuned_np = 3;
 	}
@@ -306,6 +321,14 @@ int ct_engine_set_row_form(ct_engine *e, int form)
 	return CT_OK;
 }
 
+int ct_engine_set_prime_halves(ct_engine *e, int on)
+{
+	if (!e) return fail(CT_ERR_ARG, "null engine");
+	if (e->kind != CT_KIND_RESIDENT) return fail(CT_ERR_UNSUPPORTED, "prime halves belong to the resident kernels");
+	e->res_halves = (on != 0);
+	return CT_OK;
+}
+
 int ct_engine_set_column_split(ct_engine *e, int on)
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
@@ -318,7 +341,7 @@ int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused
 {
 	if (!e) return fail(CT_ERR_ARG, "null engine");
 	if (col_w) *col_w = (e->kind == CT_KIND_PLAIN) ? e->col_w : 0;
-	if (split) *split = (e->kind == CT_KIND_PLAIN && e->split_a) ? 1 : 0;
+	if (split) *split = ((e->kind == CT_KIND_PLAIN && e->split_a) || (e->kind == CT_KIND_RESIDENT && e->res_halves)) ? 1 : 0;
 	if (fused) *fused = (e->kind == CT_KIND_PLAIN && e->use_fused) ? 1 : 0;
 	if (row_form) *row_form = (e->kind == CT_KIND_PLAIN) ? e->row_form : 0;
 	if (ms) for (int i = 0; i < CT_TUNE_TIMINGS; ++i) ms[i] = e->tune_ms[i];
@@ -543,7 +566,7 @@ int ct::steps(ct_engine *e, const uint32_t *bits, uint32_t bit_words, int i_hi, 
 	A.x = e->reg(0); A.tw = e->d_tw; A.dv = e->d_dv;
 	A.bits = bits; A.bit_words = bit_words; A.mask = mask_words; A.lane_i = lane_i;
 	A.i_hi = i_hi; A.nsteps = nsteps; A.gc = e->gc; A.base_m1 = (int)e->base - 1;
-	HIPCHK(resident_steps((int)e->tln, e->np, e->sign < 0, A, e->lanes, e->st));
+	HIPCHK(resident_steps((int)e->tln, e->np, e->sign < 0, e->res_halves, A, e->lanes, e->st));
 	return CT_OK;
 }
 

@@ -35,6 +35,7 @@ struct ct_engine
 	uint32_t rows_log2 = 0;           // column length R = 2^rows_log2
 	uint32_t col_w = 0;               // column-pass tile width (0 = default for the length)
 	bool split_a = false;             // column pass A: one workgroup per column group and prime
+	bool res_halves = false;          // resident kernels: two primes in the halves of one workgroup
 	int row_form = 0;                 // row pass B: 0 a workgroup per (row, prime), 1 per row with its primes
 	                                  // in sequence, 2 per row with its primes in workgroup slices
 	bool use_fused = false;           // fuse column pass C with the next pass A
@@ -49,7 +50,7 @@ struct ct_engine
 	int64_t *d_xc = nullptr;          // fused pass: last-chunk carries per (lane, column group, row)
 	// autotuner timings (ms per squaring): resident, multipass W=16, W=32, and
 	// the same two tiles with column pass A split by prime
-	float tune_ms[CT_TUNE_TIMINGS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+	float tune_ms[CT_TUNE_TIMINGS] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 	bool tuned = false;               // created with csize 0 (autotuned)
 	int tuned_np = 0;                 // prime count the multipass parameters were tuned for
 	bool pending = false;             // register 0 still owes the chunk carries (plain engine)

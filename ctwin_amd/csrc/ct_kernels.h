@@ -36,7 +36,10 @@ struct mul_args
 	garner_consts gc;
 };
 
-hipError_t resident_steps(int tln, int np, bool minus, const res_args &A, uint32_t lanes, hipStream_t st);
+// halves: the two primes in the halves of one workgroup (two primes and
+// resident_halves_ok(tln) only; otherwise ignored)
+hipError_t resident_steps(int tln, int np, bool minus, bool halves, const res_args &A, uint32_t lanes, hipStream_t st);
+bool resident_halves_ok(int tln);
 hipError_t resident_mul(int tln, int np, bool minus, const mul_args &A, uint32_t lanes, hipStream_t st);
 
 }   // namespace ct

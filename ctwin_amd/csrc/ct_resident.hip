@@ -206,7 +206,105 @@ __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool du
 	__syncthreads();
 }
 
+// Two-prime squaring with the primes in the two halves of a workgroup of 2T
+// threads (thread tf: prime tf / T, role t = tf % T): each half runs its
+// prime's transform with its own LDS region (both halves pass the same
+// barriers, the halves being whole wavefronts), writes its residues to that
+// region in layout 0, and the first half runs the CRT and the chunked carry
+// on both primes' residues as step does.  Twice the wavefronts per candidate
+// for n = 2^10 .. 2^13, the same digits.
+template <int LN>
+struct res_halves
+{
+	static constexpr bool ok = (geo<LN>::T % 64 == 0) && (2 * geo<LN>::T <= 1024);
+};
+
+template <int LN, bool MINUS>
+__device__ __forceinline__ void step_h(int32_t *D, uint32_t P, bool dup, int base_m1, const divisor &dv,
+                                       uint32_t (*lds)[geo<LN>::lds_words], int64_t *fbuf, const tw_entry *tw,
+                                       const garner_consts &gc, uint32_t t)
+{
+	constexpr int a = geo<LN>::a;
+	constexpr uint32_t T = geo<LN>::T;
+	constexpr uint32_t TC = MINUS ? T / 2 : T;
+	int32_t none[E];
+#pragma unroll
+	for (int q = 0; q < E; ++q) none[q] = 0;
+	uint32_t r[E];
+	if (P == 0) transform<LN, 0, 2, false, MINUS>(D, none, r, lds[0], tw, t);
+	else transform<LN, 1, 2, false, MINUS>(D, none, r, lds[1], tw, t);
+	// both primes' residues, layout 0, each in its own region
+	__syncthreads();
+	{
+		uint32_t *w = lds[P] + lbase<LN, 0>(t);
+#pragma unroll
+		for (int q = 0; q < E; ++q) w[loff<LN, 0>(q)] = r[q];
+	}
+	__syncthreads();
+	int32_t dig[E];
+	if (P == 0 && t < TC)
+	{
+		uint32_t r0[E], r1[E];
+		const uint32_t *s0 = lds[0] + lbase<LN, a>(t), *s1 = lds[1] + lbase<LN, a>(t);
+#pragma unroll
+		for (int q = 0; q < E; ++q) { r0[q] = s0[loff<LN, a>(q)]; r1[q] = s1[loff<LN, a>(q)]; }
+		int64_t l[E];
+#pragma unroll
+		for (int q = 0; q < E; ++q) l[q] = garner2(r0[q], r1[q], gc);
+		fbuf[t] = carry_chunk64(l, dig, dup, base_m1, dv);
+	}
+	__syncthreads();
+	if (P == 0 && t < TC)
+	{
+		int64_t c = fbuf[(t + TC - 1) % TC];
+		if (t == 0) c = MINUS ? c : -c;
+		else if (t == TC / 2) c += fbuf[TC - 1];
+		carry_in(dig, c, dv);
+		int32_t *dw = D + lbase<LN, a>(t);
+#pragma unroll
+		for (int q = 0; q < E; ++q) dw[loff<LN, a>(q)] = dig[q];
+	}
+	__syncthreads();
+}
+
 // ---- kernels -----------------------------------------------------------------
+
+template <int LN, bool MINUS>
+__global__ __launch_bounds__(2 * geo<LN>::T) void k_res_steps_h(res_args A)
+{
+	static_assert(res_halves<LN>::ok, "halves of whole wavefronts, at most 1024 threads");
+	constexpr uint32_t T = geo<LN>::T, n = MINUS ? geo<LN>::n / 2 : geo<LN>::n;
+	constexpr int QD = MINUS ? E / 2 : E;
+	__shared__ uint32_t lds[2][geo<LN>::lds_words];
+	__shared__ int32_t D[geo<LN>::lds_words];
+	__shared__ int64_t fbuf[T];
+	const uint32_t c = blockIdx.x, tf = threadIdx.x, P = tf / T, t = tf % T;
+	const int i_top = A.lane_i ? A.lane_i[c] : A.i_hi;
+	if (A.lane_i && i_top < 0) return;          // idle lane (uniform per workgroup)
+	int32_t *x = A.x + (size_t)c * n;
+	if (P == 0)
+	{
+		int32_t *dw = D + lbase<LN, 0>(t);
+#pragma unroll
+		for (int q = 0; q < E; ++q) dw[loff<LN, 0>(q)] = (q < QD) ? x[t + q * T] : 0;
+	}
+	__syncthreads();
+	const divisor dv = A.dv[c];
+	const uint32_t *bits = A.bits ? A.bits + (size_t)c * A.bit_words : nullptr;
+	const bool mdup = A.mask ? ((A.mask[c >> 5] >> (c & 31)) & 1u) : false;
+	for (int it = 0; it < A.nsteps; ++it)
+	{
+		const int i = i_top - it;
+		const bool dup = bits ? ((bits[i >> 5] >> (i & 31)) & 1u) : mdup;
+		step_h<LN, MINUS>(D, P, dup, A.base_m1, dv, lds, fbuf, A.tw, A.gc, t);
+	}
+	if (P == 0)
+	{
+		const int32_t *dr = D + lbase<LN, 0>(t);
+#pragma unroll
+		for (int q = 0; q < QD; ++q) x[t + q * T] = dr[loff<LN, 0>(q)];
+	}
+}
 
 template <int LN, int NP, bool MINUS>
 __global__ __launch_bounds__(geo<LN>::T) void k_res_steps(res_args A)
@@ -273,9 +371,17 @@ __global__ __launch_bounds__(geo<LN>::T) void k_res_mul(mul_args A)
 }
 
 template <int LN>
-static hipError_t launch_steps_ln(int np, bool minus, const res_args &A, uint32_t lanes, hipStream_t st)
+static hipError_t launch_steps_ln(int np, bool minus, bool halves, const res_args &A, uint32_t lanes, hipStream_t st)
 {
 	const dim3 g(lanes), b(geo<LN>::T);
+	if constexpr (res_halves<LN>::ok)
+		if (halves && np == 2)
+		{
+			const dim3 bh(2 * geo<LN>::T);
+			if (minus) hipLaunchKernelGGL((k_res_steps_h<LN, true>), g, bh, 0, st, A);
+			else hipLaunchKernelGGL((k_res_steps_h<LN, false>), g, bh, 0, st, A);
+			return hipGetLastError();
+		}
 	if (minus)
 	{
 		if (np == 2) hipLaunchKernelGGL((k_res_steps<LN, 2, true>), g, b, 0, st, A);
@@ -300,21 +406,25 @@ static hipError_t launch_mul_ln(int np, bool minus, const mul_args &A, uint32_t 
 	return hipGetLastError();
 }
 
-// tln: log2 of the transform length (n digits for N+, 2n for N-)
-hipError_t resident_steps(int tln, int np, bool minus, const res_args &A, uint32_t lanes, hipStream_t st)
+// tln: log2 of the transform length (n digits for N+, 2n for N-); halves:
+// two primes in the halves of one workgroup where the halves are whole
+// wavefronts (resident_halves_ok), else the one-prime-after-the-other kernel
+bool resident_halves_ok(int tln) { return tln >= 10 && tln <= 13; }
+
+hipError_t resident_steps(int tln, int np, bool minus, bool halves, const res_args &A, uint32_t lanes, hipStream_t st)
 {
 	switch (tln)
 	{
-	case 5: return launch_steps_ln<5>(np, minus, A, lanes, st);
-	case 6: return launch_steps_ln<6>(np, minus, A, lanes, st);
-	case 7: return launch_steps_ln<7>(np, minus, A, lanes, st);
-	case 8: return launch_steps_ln<8>(np, minus, A, lanes, st);
-	case 9: return launch_steps_ln<9>(np, minus, A, lanes, st);
-	case 10: return launch_steps_ln<10>(np, minus, A, lanes, st);
-	case 11: return launch_steps_ln<11>(np, minus, A, lanes, st);
-	case 12: return launch_steps_ln<12>(np, minus, A, lanes, st);
-	case 13: return launch_steps_ln<13>(np, minus, A, lanes, st);
-	case 14: return launch_steps_ln<14>(np, minus, A, lanes, st);
+	case 5: return launch_steps_ln<5>(np, minus, halves, A, lanes, st);
+	case 6: return launch_steps_ln<6>(np, minus, halves, A, lanes, st);
+	case 7: return launch_steps_ln<7>(np, minus, halves, A, lanes, st);
+	case 8: return launch_steps_ln<8>(np, minus, halves, A, lanes, st);
+	case 9: return launch_steps_ln<9>(np, minus, halves, A, lanes, st);
+	case 10: return launch_steps_ln<10>(np, minus, halves, A, lanes, st);
+	case 11: return launch_steps_ln<11>(np, minus, halves, A, lanes, st);
+	case 12: return launch_steps_ln<12>(np, minus, halves, A, lanes, st);
+	case 13: return launch_steps_ln<13>(np, minus, halves, A, lanes, st);
+	case 14: return launch_steps_ln<14>(np, minus, halves, A, lanes, st);
 	default: return hipErrorInvalidValue;
 	}
 }

@@ -47,9 +47,10 @@ typedef struct ct_engine ct_engine;
  * n <= 2^14, else multipass).  csize = 0 runs the autotuner first, as the
  * reference does for csize 0 (transform.h:383-419, which times CSIZE 8..64):
  * it times squarings under each compiled configuration -- resident kernels
- * vs the multipass engine when kind is AUTO and n <= 2^14, and for the
- * multipass engine the column tile, the split of column pass A and the fused
- * column pass -- keeps the fastest, and caches the choice per (device, n,
+ * vs the multipass engine when kind is AUTO and n <= 2^14; for the resident
+ * kernels, two primes one after the other or in workgroup halves; for the
+ * multipass engine the column tile, the split of column pass A, the fused
+ * column pass and the row-pass form -- keeps the fastest, and caches the choice per (device, n,
  * lanes, kind, sign, prime count) for the process.  The engine is tuned for
  * three primes at creation; when ct_engine_set_b or ct_engine_set_primes
  * moves a tuned multipass engine to another prime count, its multipass
@@ -97,14 +98,24 @@ int ct_engine_set_fused(ct_engine *e, int on);
  * form 0).  Results do not depend on it.  No counterpart in the reference. */
 int ct_engine_set_row_form(ct_engine *e, int form);
 
+/* Resident engine: run the two primes of a two-prime squaring in the two
+ * halves of one workgroup twice the size (on = 1), for transform lengths
+ * 2^10 .. 2^13, where each half is whole wavefronts; the first half then runs
+ * the CRT and the carry.  Ignored for other lengths and with three primes.
+ * Results do not depend on it.  No counterpart in the reference. */
+int ct_engine_set_prime_halves(ct_engine *e, int on);
+
 /* What the autotuner chose and measured: the column tile (0 for resident
  * kernels), the split of the column passes, the fused column pass, the row
  * form (0, 1 or 2, ct_engine_set_row_form), and ms per squaring (-1 where not
  * timed or when the engine was not tuned) in ms[CT_TUNE_TIMINGS]: {resident,
  * multipass W=16, W=32, W=16 split, W=32 split, W=16 fused, W=32 fused, W=16
  * fused split, W=32 fused split, the fastest of these with row form 1, with
- * row form 2}.  Any output may be null. */
-#define CT_TUNE_TIMINGS 11
+ * row form 2, resident with two primes one after the other, resident with
+ * two primes in workgroup halves}.  For resident engines, split reports the
+ * primes in workgroup halves (ct_engine_set_prime_halves).  Any output may be
+ * null. */
+#define CT_TUNE_TIMINGS 13
 int ct_engine_tuning(const ct_engine *e, uint32_t *col_w, int *split, int *fused, int *row_form, float *ms);
 int ct_engine_destroy(ct_engine *e);
 

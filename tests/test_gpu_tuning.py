@@ -30,14 +30,16 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     tuned = ct.Transform(ln, lanes, ct.KIND_AUTO, csize=0)
     w, split, fused, ms, rs = tuned.tuning()
     assert len(ms) == ct.TUNE_TIMINGS
-    timed = [m for m in ms if m > 0]
+    timed = [m for m in ms[:11] if m > 0]       # the kind and multipass choices (three primes)
     assert timed, ms
     # the fused column pass is timed exactly for N+ with 256-row columns
     assert (ms[5] > 0) == (ln >= 13 and ms[1] > 0)
     # the chosen configuration is the fastest one timed
     best = min(timed)
     if tuned.kind == ct.KIND_RESIDENT:
-        assert ms[0] == best and w == 0 and not split and not fused and not rs
+        assert ms[0] == best and w == 0 and not fused and not rs
+        # two primes in workgroup halves where timed faster (2^10 .. 2^13)
+        assert split == (ms[12] > 0 and ms[12] < ms[11])
     else:
         assert w in (16, 32, 64)
         if fused:
@@ -53,7 +55,7 @@ def test_tuned_engine_matches_untuned(ln, lanes):
     # tuned again for that count and runs the fastest configuration timed
     w2, split2, fused2, ms2, rs2 = tuned.tuning()
     if tuned.kind == ct.KIND_PLAIN:
-        timed2 = [m for m in ms2[1:] if m > 0]
+        timed2 = [m for m in ms2[1:11] if m > 0]
         idx2 = ((5 if w2 == 16 else 6) if fused2 else (1 if w2 in (16, 64) else 2)) + (2 if split2 else 0)
         assert ms2[8 + rs2 if rs2 else idx2] == min(timed2)
     # second creation hits the cache: same choice, same timings
