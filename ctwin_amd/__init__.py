@@ -30,7 +30,7 @@ EXPORTS = ["ct_engine_create", "ct_engine_destroy", "ct_engine_set_b", "ct_engin
            "ct_engine_is_prime", "ct_engine_read_state", "ct_engine_write_state", "ct_engine_set_exponents",
            "ct_engine_run", "ct_prp_check", "ct_engine_timer_start", "ct_engine_timer_stop", "ct_engine_info",
            "ct_sieve_window", "ct_last_error", "ct_engine_set_primes", "ct_engine_primes",
-           "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_step", "ct_server_poll",
+           "ct_server_create", "ct_server_destroy", "ct_server_submit", "ct_server_submit_ex", "ct_server_step", "ct_server_poll",
            "ct_server_stats", "ct_sieve_factors", "ct_engine_moduli",
            "ct_engine_set_graphs", "ct_engine_set_column_tile", "ct_engine_tuning", "ct_crc32", "ct_sieve_save",
            "ct_sieve_load", "ct_context_save", "ct_context_load", "ct_cyclo_check_file", "ct_cyclo_quit", "ct_server_set_graphs",
@@ -112,6 +112,7 @@ def lib():
             "ct_server_create": [i32, u64, u32, u32, u32, c.POINTER(vp)],
             "ct_server_destroy": [vp],
             "ct_server_submit": [vp, u64, u32, u32, u32],
+            "ct_server_submit_ex": [vp, u64, u32, u32, u32, i32],
             "ct_server_step": [vp],
             "ct_server_poll": [vp, u64p, u64p, u64p, np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS"), u32,
                                c.POINTER(u32)],
@@ -415,8 +416,9 @@ def residue_checks(canon, bs, qs):
 
 
 class Server:
-    """Continuous-batching PRP server (config 5): submit(id, log2_n, b, steps),
-    step() rounds, poll() finished (id, res64, hash, latency)."""
+    """Continuous-batching PRP server (config 5): submit(id, log2_n, b, steps,
+    sign=+1 for N+ or -1 for N-), step() rounds, poll() finished (id, res64,
+    hash, latency)."""
 
     def __init__(self, hbm_budget=8 << 30, slots=64, chunk=250, max_steps=4096, device=0):
         h = ctypes.c_void_p()
@@ -434,8 +436,11 @@ class Server:
     def set_graphs(self, on):
         _check(lib().ct_server_set_graphs(self.h, int(bool(on))))
 
-    def submit(self, rid, log2_n, b, steps):
-        _check(lib().ct_server_submit(self.h, int(rid), int(log2_n), int(b), int(steps)))
+    def submit(self, rid, log2_n, b, steps, sign=1):
+        if sign == 1:
+            _check(lib().ct_server_submit(self.h, int(rid), int(log2_n), int(b), int(steps)))
+        else:
+            _check(lib().ct_server_submit_ex(self.h, int(rid), int(log2_n), int(b), int(steps), int(sign)))
 
     def step(self):
         _check(lib().ct_server_step(self.h))

@@ -1,20 +1,23 @@
 // Serving loop: continuous batching of PRP-loop requests of mixed sizes.
 //
-// A request (id, n = 2^ln, b, t) asks for x_t = 2^(top t bits of E) mod N,
-// N = b^n - b^(n/2) + 1, E = N - 1: t squarings of the PRP loop of
-// cyclo.h:277-308 from x = 1.  Requests are grouped by n into size classes;
-// each class owns one engine with `slots` lanes (resident kernels for
-// n <= 2^14, the multipass engine above) and its own HIP stream.  A request is
+// A request (id, n = 2^ln, b, t, sign) asks for x_t = 2^(top t bits of E) mod
+// N, N = b^n - b^(n/2) + 1 (sign +1) or b^n - b^(n/2) - 1 (sign -1), E = N - 1:
+// t squarings of the PRP loop of cyclo.h:277-308 from x = 1.  Requests are
+// grouped by (n, sign) into size classes; each class owns one engine with
+// `slots` lanes (resident kernels for transforms of up to 2^14 points -- N-
+// runs its transform at length 2n -- the multipass engine above) and its own
+// HIP stream.  A request is
 // admitted into a free lane of its class as soon as one frees up (its digits,
 // b, exponent window and bit offset are written for that lane only; the other
 // lanes keep running), and a new class engine is created only while the sum
 // of the engines' device allocations stays within the HBM budget; idle classes
 // are released when a queued request needs the memory.  Every class runs on
-// its own cadence: a chunk of min(chunk / 2^(ln - 14) (at least 4), shortest
-// remaining) squarings is launched on the class stream with an event behind it, and the class is read
-// out, re-admitted and relaunched as soon as that event has completed, while
-// the chunks of the other classes are still running (their kernels overlap on
-// the device).  A small class is thus never held back by a large one.
+// its own cadence: a chunk of min(chunk / 2^(tl - 14) (at least 4), shortest
+// remaining) squarings, 2^tl the transform length, is launched on the class
+// stream with an event behind it, and the class is read out, re-admitted and
+// relaunched as soon as that event has completed, while the chunks of the
+// other classes are still running (their kernels overlap on the device).  A
+// small class is thus never held back by a large one.
 #include <hip/hip_runtime.h>
 #include <gmp.h>
 
@@ -39,7 +42,7 @@ double now_s()
 	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-struct request { uint64_t id; uint32_t ln, b, steps; double t_submit; };
+struct request { uint64_t id; uint32_t ln, b, steps; int sign; double t_submit; };
 
 struct lane_state { bool busy = false; request req{}; int remaining = 0; };
 
@@ -47,6 +50,7 @@ struct size_class
 {
 	ct_engine *e = nullptr;
 	uint32_t ln = 0, words = 0;
+	int sign = 1;
 	int32_t *d_lane_i = nullptr;
 	std::vector<int32_t> lane_i;
 	std::vector<lane_state> lanes;
@@ -62,9 +66,10 @@ struct size_class
 
 struct result { uint64_t id, res64, hash; double latency; };
 
-// Top k bits of E = b^n - b^(n/2): w = E >> s (certified truncated squaring of
-// b; exact arithmetic when E is short).  Returns false when not certified.
-bool exponent_window(uint32_t b, uint32_t n, uint32_t k, mpz_t w)
+// Top k bits of E = b^n - b^(n/2) (sign +1) or b^n - b^(n/2) - 2 (sign -1):
+// w = E >> s (certified truncated squaring of b; exact arithmetic when E is
+// short).  Returns false when not certified.
+bool exponent_window(uint32_t b, uint32_t n, uint32_t k, int sign, mpz_t w)
 {
 	const uint64_t ebits = (uint64_t)n * 32;
 	if (ebits < 4 * (uint64_t)k + 4096)
@@ -72,6 +77,7 @@ bool exponent_window(uint32_t b, uint32_t n, uint32_t k, mpz_t w)
 		mpz_t h, E; mpz_inits(h, E, NULL);
 		mpz_ui_pow_ui(h, b, n / 2);
 		mpz_mul(E, h, h); mpz_sub(E, E, h);
+		if (sign < 0) mpz_sub_ui(E, E, 2);
 		const size_t len = mpz_sizeinbase(E, 2);
 		if (len > k) mpz_fdiv_q_2exp(w, E, len - k); else mpz_set(w, E);
 		mpz_clears(h, E, NULL);
@@ -96,20 +102,28 @@ bool exponent_window(uint32_t b, uint32_t n, uint32_t k, mpz_t w)
 	const size_t sl = mpz_sizeinbase(hi, 2) - k;
 	mpz_fdiv_q_2exp(w, lo, sl); mpz_fdiv_q_2exp(t, hi, sl);
 	ok = ok && mpz_cmp(w, t) == 0;
+	// b^n = w 2^(e + sl) + r with r >= t 2^e, t = lo mod 2^sl; subtracting
+	// b^(n/2) (+ 2) < 2^(h + 1), h = (n/2) bits(b), leaves the window alone
+	// when t 2^e >= 2^(h + 1)
 	const uint64_t bl = 64 - __builtin_clzll((uint64_t)b);
 	mpz_fdiv_r_2exp(t, lo, sl);
-	ok = ok && (e + sl > (uint64_t)(n / 2) * bl + 64) && mpz_sgn(t) != 0;
+	ok = ok && mpz_sgn(t) != 0 && e + mpz_sizeinbase(t, 2) - 1 >= (uint64_t)(n / 2) * bl + 1;
 	mpz_clears(lo, hi, t, NULL);
 	return ok;
 }
 
-uint64_t class_bytes(uint32_t ln, uint32_t slots, uint32_t words, bool multipass)
+// Device bytes of a class: digit registers (n per lane), tables and residues
+// (transform length tn), exponent words, carries.
+uint64_t class_bytes(uint32_t ln, int sign, uint32_t slots, uint32_t words, bool multipass)
 {
-	const uint64_t n = 1ull << ln;
-	uint64_t b = 3 * (uint64_t)slots * n * 4 + 48 * n + (uint64_t)slots * (words * 4 + 64);
-	if (multipass) b += 2 * 3 * (uint64_t)slots * n * 4 + (uint64_t)slots * (n / 16) * 8;
+	const uint64_t n = 1ull << ln, tn = sign < 0 ? 2 * n : n;
+	uint64_t b = 3 * (uint64_t)slots * n * 4 + 48 * tn + (uint64_t)slots * (words * 4 + 64);
+	if (multipass) b += 2 * 3 * (uint64_t)slots * tn * 4 + (uint64_t)slots * (n / 16) * 8;
 	return b;
 }
+
+// Size classes are keyed by (n, sign).
+uint32_t class_key(uint32_t ln, int sign) { return 2 * ln + (sign < 0 ? 1 : 0); }
 
 }   // namespace
 
@@ -125,9 +139,9 @@ struct ct_server
 	bool graphs = true;          // multipass classes replay captured graphs
 };
 
-static void destroy_class(ct_server *s, uint32_t ln)
+static void destroy_class(ct_server *s, uint32_t key)
 {
-	auto it = s->classes.find(ln);
+	auto it = s->classes.find(key);
 	if (it == s->classes.end()) return;
 	size_class &c = *it->second;
 	if (c.e && c.e->st) (void)hipStreamSynchronize(c.e->st);
@@ -155,28 +169,35 @@ int ct_server_destroy(ct_server *s)
 	return CT_OK;
 }
 
-int ct_server_submit(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps)
+int ct_server_submit_ex(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps, int sign)
 {
 	if (!s) return fail(CT_ERR_ARG, "null server");
-	if (log2_n < 10 || log2_n > 22) return fail(CT_ERR_UNSUPPORTED, "served sizes are 2^10 .. 2^22");
+	if (sign != 1 && sign != -1) return fail(CT_ERR_ARG, "sign must be +1 or -1");
+	if (log2_n < 10 || log2_n > (sign > 0 ? 22u : 21u))
+		return fail(CT_ERR_UNSUPPORTED, "served sizes are 2^10 .. 2^22 (N+), 2^10 .. 2^21 (N-)");
 	if (b <= 32 || b >= 0x80000000u) return fail(CT_ERR_ARG, "b must satisfy 32 < b < 2^31");
 	if (steps == 0 || steps > s->max_steps) return fail(CT_ERR_ARG, "steps out of range");
-	s->queue.push_back(request{id, log2_n, b, steps, now_s()});
+	s->queue.push_back(request{id, log2_n, b, steps, sign, now_s()});
 	return CT_OK;
 }
 
-static int make_class(ct_server *s, uint32_t ln, size_class **out)
+int ct_server_submit(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps)
+{
+	return ct_server_submit_ex(s, id, log2_n, b, steps, 1);
+}
+
+static int make_class(ct_server *s, uint32_t ln, int sign, size_class **out)
 {
 	const uint32_t words = (s->max_steps + 31) / 32 + 1;
 	// admission is checked against the larger (multipass) footprint; the
 	// autotuned engine kind then fixes what is charged
-	if (s->used + class_bytes(ln, s->slots, words, true) > s->budget) return CT_ERR_NOMEM;
+	if (s->used + class_bytes(ln, sign, s->slots, words, true) > s->budget) return CT_ERR_NOMEM;
 	std::unique_ptr<size_class> c(new size_class());
-	int rc = ct_engine_create(s->device, ln, s->slots, 0, CT_KIND_AUTO, &c->e);
+	int rc = ct_engine_create_ex(s->device, ln, s->slots, 0, CT_KIND_AUTO, sign, &c->e);
 	if (rc) return rc;
-	const uint64_t bytes = class_bytes(ln, s->slots, words, c->e->kind == CT_KIND_PLAIN);
+	const uint64_t bytes = class_bytes(ln, sign, s->slots, words, c->e->kind == CT_KIND_PLAIN);
 	c->e->use_graphs = c->e->use_graphs && s->graphs;
-	c->ln = ln; c->words = words; c->bytes = bytes;
+	c->ln = ln; c->sign = sign; c->words = words; c->bytes = bytes;
 	c->lanes.resize(s->slots);
 	c->lane_i.assign(s->slots, -1);
 	// every lane gets a valid b before the first launch (idle lanes never run)
@@ -195,7 +216,7 @@ static int make_class(ct_server *s, uint32_t ln, size_class **out)
 	c->stage_lane_i.assign(s->slots, -1);
 	s->used += bytes;
 	*out = c.get();
-	s->classes[ln] = std::move(c);
+	s->classes[class_key(ln, sign)] = std::move(c);
 	return CT_OK;
 }
 
@@ -205,7 +226,7 @@ static int admit(size_class &c, uint32_t l, const request &r)
 	ct_engine *e = c.e;
 	const uint32_t n = e->n;
 	mpz_t w; mpz_init(w);
-	if (!exponent_window(r.b, n, r.steps, w)) { mpz_clear(w); return fail(CT_ERR_COMPUTE, "exponent window not certified"); }
+	if (!exponent_window(r.b, n, r.steps, r.sign, w)) { mpz_clear(w); return fail(CT_ERR_COMPUTE, "exponent window not certified"); }
 	uint32_t *bits = &c.stage_bits[(size_t)l * c.words];
 	std::memset(bits, 0, c.words * sizeof(uint32_t));
 	size_t cnt = 0;
@@ -257,7 +278,7 @@ static int complete_class(ct_server *s, size_class &c)
 		if (!(ls.busy && ls.remaining == 0)) continue;
 		if (hipMemcpy(raw.data(), e->reg(0) + (size_t)l * n, n * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
 			return fail(CT_ERR_HIP, "read-out");
-		if (canonical(raw.data(), n, ls.req.b, 1, canon.data())) return fail(CT_ERR_GETINT, "getInt failed");
+		if (canonical(raw.data(), n, ls.req.b, c.sign, canon.data())) return fail(CT_ERR_GETINT, "getInt failed");
 		uint64_t r64 = canon[0], bi = ls.req.b, h = 1469598103934665603ull;
 		for (uint32_t j = 1; j < n; ++j) { r64 += (uint64_t)canon[j] * bi; bi *= ls.req.b; }
 		for (uint32_t j = 0; j < n; ++j) { h ^= canon[j]; h *= 1099511628211ull; }
@@ -297,11 +318,12 @@ static int poll_classes(ct_server *s, bool wait)
 static int launch_class(ct_server *s, size_class &c)
 {
 	if (c.in_flight) return CT_OK;
-	// chunks sized by cost: a class of 2^ln digits launches chunk / 2^(ln - 14)
-	// squarings (at least 4), so that a large class's launch sequences stay
-	// short and the kernels of small classes that share its hardware queue
-	// are not held behind them
-	int k = std::max(4, (int)(s->chunk >> (c.ln > 14 ? c.ln - 14 : 0)));
+	// chunks sized by cost: a class whose transform has 2^tl points launches
+	// chunk / 2^(tl - 14) squarings (at least 4), so that a large class's
+	// launch sequences stay short and the kernels of small classes that share
+	// its hardware queue are not held behind them
+	const uint32_t tl = c.e->tln;
+	int k = std::max(4, (int)(s->chunk >> (tl > 14 ? tl - 14 : 0)));
 	bool any = false;
 	for (auto &ls : c.lanes) if (ls.busy) { any = true; k = std::min(k, ls.remaining); }
 	if (!any) return CT_OK;
@@ -331,11 +353,11 @@ int ct_server_step(ct_server *s)
 	for (auto it = s->queue.begin(); it != s->queue.end();)
 	{
 		const request r = *it;
-		auto ci = s->classes.find(r.ln);
+		auto ci = s->classes.find(class_key(r.ln, r.sign));
 		size_class *c = (ci != s->classes.end()) ? ci->second.get() : nullptr;
 		if (!c)
 		{
-			rc = make_class(s, r.ln, &c);
+			rc = make_class(s, r.ln, r.sign, &c);
 			if (rc == CT_ERR_NOMEM)
 			{   // release idle classes and retry once
 				std::vector<uint32_t> idle;
@@ -345,8 +367,8 @@ int ct_server_step(ct_server *s)
 					for (auto &ls : kv.second->lanes) any |= ls.busy;
 					if (!any) idle.push_back(kv.first);
 				}
-				for (uint32_t ln : idle) destroy_class(s, ln);
-				rc = make_class(s, r.ln, &c);
+				for (uint32_t key : idle) destroy_class(s, key);
+				rc = make_class(s, r.ln, r.sign, &c);
 			}
 			if (rc == CT_ERR_NOMEM)
 			{

@@ -238,13 +238,20 @@ int ct_engine_moduli(uint32_t *p);
  * yields x = 2^(top `steps` bits of N - 1) mod N, N = b^n - b^(n/2) + 1.
  * Requests are admitted into free lanes of per-size engines as lanes free up;
  * a new size class is created only within hbm_budget bytes of device memory.
- * slots = lanes per size class; chunk = squarings per launch of a class of up
- * to 2^14 digits (larger classes launch chunk / 2^(log2_n - 14), at least 4);
- * max_steps bounds `steps`.  10 <= log2_n <= 22. */
+ * slots = lanes per size class; chunk = squarings per launch of a class whose
+ * transform has up to 2^14 points (larger transforms, of 2^tl points, launch
+ * chunk / 2^(tl - 14), at least 4; the transform of an N- class is twice its
+ * digit count); max_steps bounds `steps`.  10 <= log2_n <= 22. */
 typedef struct ct_server ct_server;
 int ct_server_create(int device, uint64_t hbm_budget, uint32_t slots, uint32_t chunk, uint32_t max_steps, ct_server **out);
 int ct_server_destroy(ct_server *s);
 int ct_server_submit(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps);
+/* ct_server_submit for either twin member: sign = +1 is the request above,
+ * sign = -1 asks for x = 2^(top `steps` bits of N - 1) mod N with
+ * N = b^n - b^(n/2) - 1 (10 <= log2_n <= 21), in size classes of their own
+ * (engines of ct_engine_create_ex).  No counterpart in the reference: its
+ * batch loop tests N+ only, and its sieve defines N- (ctsieve.cpp:728-787). */
+int ct_server_submit_ex(ct_server *s, uint64_t id, uint32_t log2_n, uint32_t b, uint32_t steps, int sign);
 /* One step: read out the size classes whose chunk has completed, admit
  * queued requests into free lanes of classes that are not running a chunk,
  * launch the next chunk on every idle class with work (their streams

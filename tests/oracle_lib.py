@@ -141,11 +141,11 @@ def top_bits_of_exponent(b, n, k):
     s_local = top_len - k
     a, c = lo >> s_local, hi >> s_local
     assert a == c, "top bits uncertain"
-    half_bits = (n // 2) * b.bit_length() + 1
-    assert e + s_local > half_bits + 64
-    # the subtraction cannot borrow into the window unless the bits between
-    # the window and b^(n/2) are all zero: they are not (checked on lo)
-    assert (lo & ((1 << s_local) - 1)) != 0
+    # b^n = a 2^s + r with r >= t 2^e (t = lo mod 2^s_local): the subtraction
+    # of b^(n/2) < 2^((n/2) bits(b)) cannot borrow into the window when t has
+    # a bit at or above 2^((n/2) bits(b) + 1 - e)
+    t = lo & ((1 << s_local) - 1)
+    assert t != 0 and e + t.bit_length() - 1 >= (n // 2) * b.bit_length() + 1, "window too close to b^(n/2)"
     return a, e + s_local
 
 

@@ -84,3 +84,21 @@ def test_two_prime_mode_keeps_digits_far_inside_2p():
             mid = (lo + hi) // 2
             lo, hi = (mid, hi) if 3 * n * mid * mid < h2 else (lo, mid)
         assert 2 * lo < (1 << 28) < 2 * min(P)
+
+
+def test_server_submit_checks_arguments_before_any_device_call():
+    if not os.path.exists(ctwin_amd.LIB_PATH):
+        pytest.skip("libctwin_amd.so not built")
+    s = ctwin_amd.Server(hbm_budget=1 << 30, slots=1, chunk=64, max_steps=64)
+    with pytest.raises(ctwin_amd.CtError):
+        s.submit(0, 12, 40000, 10, 0)          # sign must be +1 or -1
+    with pytest.raises(ctwin_amd.CtError):
+        s.submit(0, 22, 40000, 10, -1)         # N- up to 2^21 digits
+    with pytest.raises(ctwin_amd.CtError):
+        s.submit(0, 12, 32, 10, -1)            # b > 32
+    with pytest.raises(ctwin_amd.CtError):
+        s.submit(0, 12, 40000, 65, 1)          # steps <= max_steps
+    s.submit(0, 22, 40000, 10, 1)
+    s.submit(1, 21, 40000, 10, -1)
+    assert s.stats()["queued"] == 2
+    s.close()

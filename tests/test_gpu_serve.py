@@ -19,9 +19,9 @@ def fnv(canon):
     return h
 
 
-def python_result(ln, b, steps):
+def python_result(ln, b, steps, sign=1):
     n = 1 << ln
-    E = b ** n - b ** (n // 2)
+    E = b ** n - b ** (n // 2) - (0 if sign > 0 else 2)
     N = E + 1
     x = pow(2, E >> (E.bit_length() - steps), N)
     canon = []
@@ -32,10 +32,10 @@ def python_result(ln, b, steps):
     return x % (1 << 64), fnv(canon)
 
 
-def gmp_res64(ln, b, steps):
+def gmp_res64(ln, b, steps, sign=1):
     """RES64 (x mod 2^64 of the canonical residue) of the same request by GMP."""
     n = 1 << ln
-    E = b ** n - b ** (n // 2)
+    E = b ** n - b ** (n // 2) - (0 if sign > 0 else 2)
     N = E + 1
     return O.powm(2, E >> (E.bit_length() - steps), N) % (1 << 64)
 
@@ -109,3 +109,41 @@ def test_serve_budget_forces_class_recycling():
         alone.submit(*r)
         (rid, v), = drain(alone, 1).items()
         assert got[rid][:2] == v[:2]
+
+
+def test_serve_twin_members_mixed_equal_alone_and_gmp():
+    """N+ and N- requests of the same sizes share the server: each sign has
+    its own size classes (N-: resident kernels up to n = 2^13, multipass
+    above), and every result equals the request run alone and Python pow or
+    GMP."""
+    rng = random.Random(21)
+    reqs = []
+    for i in range(28):
+        ln = rng.choice([10, 11, 12, 13, 14, 15])
+        b = rng.randrange(1 << 15, 1 << 21)
+        reqs.append((i, ln, b, 250, 1 if i % 2 else -1))
+    print(f"\n[case] serve twin members: {len(reqs)} requests", flush=True)
+    s = ct.Server(hbm_budget=4 << 30, slots=4, chunk=64, max_steps=512)
+    for r in reqs:
+        s.submit(*r)
+    mixed = {rid: v[:2] for rid, v in drain(s, len(reqs)).items()}
+    s.close()
+    alone = ct.Server(hbm_budget=4 << 30, slots=1, chunk=64, max_steps=512)
+    for r in reqs:
+        alone.submit(*r)
+        (rid, v), = drain(alone, 1).items()
+        assert mixed[rid] == v[:2], f"request {r} differs alone vs mixed"
+    alone.close()
+    for rid, ln, b, steps, sign in reqs:
+        if ln <= 12:
+            assert mixed[rid] == python_result(ln, b, steps, sign), f"request {(rid, ln, b, sign)} vs Python pow"
+        else:
+            assert mixed[rid][0] == gmp_res64(ln, b, steps, sign), f"request {(rid, ln, b, sign)} vs GMP"
+    # the two members of one b differ
+    a = ct.Server(hbm_budget=1 << 30, slots=2, chunk=64, max_steps=512)
+    a.submit(0, 12, 40000, 200, 1)
+    a.submit(1, 12, 40000, 200, -1)
+    got = drain(a, 2)
+    assert got[0][:2] == python_result(12, 40000, 200, 1)
+    assert got[1][:2] == python_result(12, 40000, 200, -1)
+    assert got[0][0] != got[1][0]
