@@ -68,12 +68,12 @@ def run_c1(ct):
     """Config 1: the full Fermat PRP test with Gerbicz-Li of the candidate
     pair N+- = b^1024 - b^512 +- 1, b even, uniform in [2^19, 2^20) (seed +
     1): base 3 (the configuration's) for both members and base 2 (the
-    reference's) for N+, as a batch of 64 (the reference's lane count) and as
-    the plain one-candidate engine; RES64 of every test against GMP
-    (tests/golden/c1_n2e10.json)."""
-    ln, lanes = 10, 64
+    reference's) for N+, as a batch of 64 (the reference's lane count), as
+    a batch of 1024 (four workgroups per CU) and as the plain one-candidate
+    engine; RES64 of every test against GMP (tests/golden/c1_n2e10.json)."""
+    ln, wide, lanes = 10, 1024, 64
     rng = random.Random(SEED + 1)
-    bs = [2 * rng.randrange(1 << 18, 1 << 19) for _ in range(lanes)]
+    bs = [2 * rng.randrange(1 << 18, 1 << 19) for _ in range(wide)]
     fx_path = os.path.join(ROOT, "tests", "golden", "c1_n2e10.json")
     fx = json.load(open(fx_path)) if os.path.exists(fx_path) else None
     if fx and [L["b"] for L in fx["lanes"]] != bs:
@@ -82,7 +82,7 @@ def run_c1(ct):
     bad = 0
     checked = 0
     for name, sign, base in (("plus_base3", 1, 3), ("minus_base3", -1, 3), ("plus_base2", 1, 2)):
-        for count in (lanes, 1):
+        for count in (wide, lanes, 1):
             t = ct.Transform(ln, count, ct.KIND_AUTO, csize=0, sign=sign)
             t.set_base(base)
             ct.prp_check(t, bs[:count])                         # warm-up (tuned engine, first launches)
@@ -90,7 +90,7 @@ def run_c1(ct):
             prm, r64, gl, _, rc = ct.prp_check(t, bs[:count])
             dt = time.perf_counter() - t0
             t.close()
-            key = f"{name}_{'batched64' if count == lanes else 'plain1'}"
+            key = f"{name}_{'plain1' if count == 1 else f'batched{count}'}"
             out[key] = {"seconds_per_batch": dt, "candidates_per_hour": 3600.0 * count / dt,
                         "gerbicz_ok": bool(gl.all()), "status": int(rc)}
             if fx:
@@ -521,8 +521,8 @@ def main():
           f"p50 {served_light['latency_p50_ms']:.1f} ms, p99 {served_light['latency_p99_ms']:.1f} ms", flush=True)
 
     c1 = run_c1(ct)
-    print(f"[bench] config 1: {c1['plus_base3_batched64']['candidates_per_hour']:.4g} N+ tests/hour (64 lanes), "
-          f"parity {c1['parity']}", flush=True)
+    print(f"[bench] config 1: {c1['plus_base3_batched1024']['candidates_per_hour']:.4g} N+ tests/hour (1024 lanes), "
+          f"{c1['plus_base3_batched64']['candidates_per_hour']:.4g} (64 lanes), parity {c1['parity']}", flush=True)
     n = 1 << 14
     per_cand = prp_work(i0)
     valu_achieved = mul32_alg(n, 2) * sq_s / 1e12
