@@ -58,23 +58,23 @@ TRAFFIC_N2E21_SQUARING = (33170 * 1.994 + 16384 + 8629 * 1.994 + 16904) * 1024  
 VALU_SLOT_PEAK_G = 1024 * 2.4 / 4
 VALU_SLOTS_PER_SQUARING = (899907218 - 18339941) / 49                # k_res_steps<14, 2>, 256 lanes: 50- minus 1-squaring launch
 VALU_SLOTS_N2E21_SQUARING = 6932104 + 6455168                   # k_rowB<21, 8> + k_colCA2<21, 8, 32>
-SIEVE_ROOTS_NEG_SLOTS = 7505592064 - 1826200320                 # k_roots_neg, config 2, its 26 launches
-# k_roots_neg per launch on config 2: the prime list it reads (FETCH x 1.994)
+SIEVE_ROOTS_NEG_SLOTS = 7505592064 - 1826200320                 # k_roots_neg, m = 12 window of 2^24 b, its 26 launches
+# k_roots_neg per launch on that window: the prime list it reads (FETCH x 1.994)
 # and the write-back of the struck byte map (WRITE), both in HBM bytes
 TRAFFIC_SIEVE_ROOTS_NEG_LAUNCH = (401214.9 / 52 * 1.994 + 750543.7 / 52) * 1024
 
 
-def run_c1(ct):
-    """Config 1: the full Fermat PRP test with Gerbicz-Li of the candidate
+def run_prp_n2e10(ct):
+    """Full Fermat PRP tests with Gerbicz-Li at n = 2^10 of the candidate
     pair N+- = b^1024 - b^512 +- 1, b even, uniform in [2^19, 2^20) (seed +
     1): base 3 (the configuration's) for both members and base 2 (the
     reference's) for N+, as a batch of 64 (the reference's lane count), as
     a batch of 1024 (four workgroups per CU) and as the plain one-candidate
-    engine; RES64 of every test against GMP (tests/golden/c1_n2e10.json)."""
+    engine; RES64 of every test against GMP (tests/golden/prp_n2e10_b2e19.json)."""
     ln, wide, lanes = 10, 1024, 64
     rng = random.Random(SEED + 1)
     bs = [2 * rng.randrange(1 << 18, 1 << 19) for _ in range(wide)]
-    fx_path = os.path.join(ROOT, "tests", "golden", "c1_n2e10.json")
+    fx_path = os.path.join(ROOT, "tests", "golden", "prp_n2e10_b2e19.json")
     fx = json.load(open(fx_path)) if os.path.exists(fx_path) else None
     if fx and [L["b"] for L in fx["lanes"]] != bs:
         fx = None
@@ -98,7 +98,7 @@ def run_c1(ct):
                     checked += 1
                     bad += ("%016X" % int(r64[l])) != fx["lanes"][l][name]["res64"]
     out["parity"] = {"checked": fx is not None, "tests": checked, "differing": bad,
-                     "against": "GMP (tests/golden/c1_n2e10.json): RES64 of base^(N-1) mod N"}
+                     "against": "GMP (tests/golden/prp_n2e10_b2e19.json): RES64 of base^(N-1) mod N"}
     return out
 
 
@@ -234,12 +234,12 @@ def run_plain(ct, ln, b, steps, warmup, graphs=True, config=None, check=False):
     return steps / (ms / 1e3), ms, cfg
 
 
-def run_sieve_config2(ct):
-    """Config 2: m = 12, b in [2^20, 2^20 + 2^24), all p < 2^32, both sides.
+def run_sieve_window(ct):
+    """Window sieve: m = 12, b in [2^20, 2^20 + 2^24), all p < 2^32, both sides.
     One untimed call first (it creates the device buffers and loads the
     kernels), then the timed call; survivors against the reference program's
     list, the work of the reference algorithm on this window
-    (tests/golden/sieve_work_config2.json) per second, and the VALU roofline of
+    (tests/golden/sieve_work_m12_window_2e24.json) per second, and the VALU roofline of
     the N- root kernel from its live launch times."""
     import hashlib
     import numpy as np
@@ -254,12 +254,12 @@ def run_sieve_config2(ct):
     surv = ct.survivors(st, b_min)
     out = {"seconds": dt, "first_call_seconds": first, "survivors": int(len(surv)), "b_window": 1 << 24,
            "p_max": 1 << 32, "b_per_s": (1 << 24) / dt, "device": timing}
-    fx = os.path.join(ROOT, "tests", "golden", "sieve_config2.json")
+    fx = os.path.join(ROOT, "tests", "golden", "sieve_m12_window_2e24.json")
     if os.path.exists(fx):
         want = json.load(open(fx))
         text = "".join(f"{int(b)}\n" for b in surv)
         out["bit_exact_vs_reference"] = hashlib.sha256(text.encode()).hexdigest() == want["sha256"]
-    wk = os.path.join(ROOT, "tests", "golden", "sieve_work_config2.json")
+    wk = os.path.join(ROOT, "tests", "golden", "sieve_work_m12_window_2e24.json")
     if os.path.exists(wk):
         work = json.load(open(wk))
         total = work["montgomery_products"] + work["verify_products"]
@@ -295,7 +295,7 @@ def sieve_pool(ct, m, bits, cache):
 
 
 def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30, graphs=True):
-    """Config 5: Poisson arrivals, n = 2^k with k uniform in 12..20, b of 16..22
+    """Served stream: Poisson arrivals, n = 2^k with k uniform in 12..20, b of 16..22
     bits taken from sieve survivors, 1000 squarings per request, admission by an
     HBM budget."""
     import numpy as np
@@ -386,7 +386,7 @@ def cpu_baseline(ct, budget_s=5.0):
       workload (n = 2^14, lanes of the bench's batch) and of the single large
       candidate (n = 2^21), one thread and every thread this process may use;
     - the reference's own CPU window sieve (oracle/_ref/ctsieve_ref, compiled
-      from sieve/CPU/ctsieve.cpp) on config 2's b window with p < 2^28, beside
+      from sieve/CPU/ctsieve.cpp) on the m = 12 window's b with p < 2^28, beside
       the GPU sieve on the same sample (same survivors required)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import hashlib
@@ -449,9 +449,9 @@ def cpu_baseline(ct, budget_s=5.0):
     else:
         sv["skipped"] = "oracle/_ref/ctsieve_ref not built"
     out["sieve"] = sv
-    out["sieve_config2_reference_s_build_container"] = {
+    out["sieve_m12_window_reference_s_build_container"] = {
         "seconds": 177.5, "where": "the build container (Xeon 2.7 GHz, 1 thread), not the GPU host",
-        "sample": "the full config 2 (p < 2^32 both sides)"}
+        "sample": "the full m = 12 window of 2^24 b (p < 2^32 both sides)"}
     return out
 
 
@@ -468,7 +468,7 @@ def main():
                     help="multipass engines launch every kernel instead of replaying captured HIP graphs "
                          "(same results; used for kernel traces)")
     ap.add_argument("--sieve-only", action="store_true",
-                    help="run only the config-2 window sieve and print its JSON object (for profiling the sieve)")
+                    help="run only the m = 12 window sieve (2^24 b, p < 2^32) and print its JSON object (for profiling the sieve)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for hardware counters, without graphs or autotuning: 50 batched steps "
                          "in one resident launch, 20 plain squarings at n=2^14 (column tile 16) and n=2^21 "
@@ -484,7 +484,7 @@ def main():
         ct.set_graphs_default(False)          # no graph anywhere in the process, the autotuner's included
 
     if args.sieve_only:
-        print(json.dumps({"sieve_config2": run_sieve_config2(ct)}), flush=True)
+        print(json.dumps({"sieve_m12_window": run_sieve_window(ct)}), flush=True)
         return
 
     if args.profile:
@@ -511,8 +511,8 @@ def main():
     big_parity = big_cfg.pop("parity")
     print(f"[bench] plain n=2^21 x1: {big_sq_s:.4g} squarings/s; parity {big_parity}", flush=True)
 
-    sieve = run_sieve_config2(ct)
-    print(f"[bench] sieve config 2: {sieve['survivors']} survivors in {sieve['seconds']:.3f} s", flush=True)
+    sieve = run_sieve_window(ct)
+    print(f"[bench] sieve m = 12, 2^24 b: {sieve['survivors']} survivors in {sieve['seconds']:.3f} s", flush=True)
     served = run_served(ct, graphs=not args.no_graphs)
     print(f"[bench] served: {served['squarings_per_s']:.4g} squarings/s, p50 {served['latency_p50_ms']:.1f} ms, "
           f"p99 {served['latency_p99_ms']:.1f} ms", flush=True)
@@ -520,9 +520,9 @@ def main():
     print(f"[bench] served below saturation: {served_light['squarings_per_s']:.4g} squarings/s, "
           f"p50 {served_light['latency_p50_ms']:.1f} ms, p99 {served_light['latency_p99_ms']:.1f} ms", flush=True)
 
-    c1 = run_c1(ct)
-    print(f"[bench] config 1: {c1['plus_base3_batched1024']['candidates_per_hour']:.4g} N+ tests/hour (1024 lanes), "
-          f"{c1['plus_base3_batched64']['candidates_per_hour']:.4g} (64 lanes), parity {c1['parity']}", flush=True)
+    prp10 = run_prp_n2e10(ct)
+    print(f"[bench] full PRP n = 2^10: {prp10['plus_base3_batched1024']['candidates_per_hour']:.4g} N+ tests/hour (1024 lanes), "
+          f"{prp10['plus_base3_batched64']['candidates_per_hour']:.4g} (64 lanes), parity {prp10['parity']}", flush=True)
     n = 1 << 14
     per_cand = prp_work(i0)
     valu_achieved = mul32_alg(n, 2) * sq_s / 1e12
@@ -564,10 +564,10 @@ def main():
                                       "note": "bound by neither: HBM traffic 1.2x the algorithmic bytes at about a "
                                               "third of 8 TB/s, VALU issue (row pass + fused column pass slots "
                                               "over the measured squaring time) a little above half"}},
-        "sieve_config2": sieve,
+        "sieve_m12_window": sieve,
         "served": served,
         "served_below_saturation": served_light,
-        "config1": c1,
+        "prp_n2e10": prp10,
         "roofline": {"bound": "valu", "achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "Tmul32/s",
                      "frac": valu_achieved / VALU_PEAK_T, "traffic": TRAFFIC_RESIDENT_LAUNCH,
                      "kernel": "k_res_steps<14, 2>", "primes": 2, "mul32_alg_per_squaring": mul32_alg(n, 2),

@@ -416,7 +416,7 @@ def residue_checks(canon, bs, qs):
 
 
 class Server:
-    """Continuous-batching PRP server (config 5): submit(id, log2_n, b, steps,
+    """Continuous-batching PRP server: submit(id, log2_n, b, steps,
     sign=+1 for N+ or -1 for N-), step() rounds, poll() finished (id, res64,
     hash, latency)."""
 

@@ -1,4 +1,4 @@
-"""Writes tests/golden/c1_n2e10.json: GMP Fermat residues of config 1, the
+"""Writes tests/golden/prp_n2e10_b2e19.json: GMP Fermat residues of full PRP tests, the
 plain PRP of the candidate pair N+- = b^1024 - b^512 +- 1 (n = 2^10 digits),
 b an even integer uniform in [2^19, 2^20) drawn from seed 20261203 + 1, for
 1024 such b (the first 64 are a batch of the reference's 64 lanes,
@@ -6,7 +6,7 @@ cyclo/src/engine.h:19; b[0] is the plain one-candidate case): RES64 of
 base^(N-1) mod N for bases 2 and 3 and both members.
 
 Test infrastructure (the oracle's mpz_powm), run in the build container:
-    python tests/golden/gen_c1_fixture.py
+    python tests/golden/gen_prp_n2e10_fixture.py
 """
 import json
 import multiprocessing
@@ -39,7 +39,7 @@ def row_for(b):
 
 
 def main():
-    path = os.path.join(HERE, "c1_n2e10.json")
+    path = os.path.join(HERE, "prp_n2e10_b2e19.json")
     # rows already in the file for the same b are kept (about 1 s of GMP per
     # residue here); the others are computed on every CPU
     have = {}

@@ -34,7 +34,7 @@ SMALL = [
      "p_max_pos": 1000000000, "p_max_neg": 100000000},
 ]
 FILES = {"m": 8, "b_min": 2, "b_max": 3001, "p_max_pos": 100000, "p_max_neg": 100000}
-FULL = {"name": "m12_config2", "m": 12, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 24) - 1,
+FULL = {"name": "m12_window_2e24", "m": 12, "b_min": 1 << 20, "b_max": (1 << 20) + (1 << 24) - 1,
         "p_max_pos": 1 << 32, "p_max_neg": 1 << 32}
 # p ranges that cross and lie above 2^32 (explicit p_min, the resume state)
 BEYOND = [
@@ -98,7 +98,7 @@ def main():
         surv, h = run(FULL)
         json.dump(dict(FULL, source="sieve/CPU/ctsieve.cpp check_pos + check_neg from the ini start",
                        count=len(surv), sha256=h, head=surv[:32], tail=surv[-32:]),
-                  open(os.path.join(HERE, "sieve_config2.json"), "w"))
+                  open(os.path.join(HERE, "sieve_m12_window_2e24.json"), "w"))
         print(FULL["name"], len(surv))
 
 

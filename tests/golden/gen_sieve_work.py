@@ -1,12 +1,12 @@
-"""Counts the arithmetic of the config-2 window sieve (m = 12,
+"""Counts the arithmetic of the 2^24-wide window sieve (m = 12,
 b in [2^20, 2^20 + 2^24), p < 2^32 on both sides) as the reference algorithm
 does it (sieve/CPU/ctsieve.cpp check_pos / check_neg / check_root, restated in
-oracle/ctwin_oracle.c), and writes tests/golden/sieve_work_config2.json:
+oracle/ctwin_oracle.c), and writes tests/golden/sieve_work_m12_window_2e24.json:
 
 - the 64-bit Montgomery products of the candidate tests, the non-residue
   searches, the square roots and the root enumeration;
 - the products of check_root's strike verification (b^(2^(m-1)) per strike);
-- the survivor count and SHA-256, checked against tests/golden/sieve_config2.json
+- the survivor count and SHA-256, checked against tests/golden/sieve_m12_window_2e24.json
   so the counted run is the one whose output is pinned.
 
 These counts are the algorithmic work of the window: the bench divides them by
@@ -25,7 +25,7 @@ import oracle_lib as O  # noqa: E402
 
 
 def main():
-    fx = json.load(open(os.path.join(HERE, "sieve_config2.json")))
+    fx = json.load(open(os.path.join(HERE, "sieve_m12_window_2e24.json")))
     m, b_min, b_max = fx["m"], fx["b_min"], fx["b_max"]
     O.sieve_products(reset=True)
     t0 = time.time()
@@ -40,7 +40,7 @@ def main():
            "montgomery_products": mont, "verify_products": verify,
            "survivors": int(len(surv)), "sha256_matches_fixture": sha == fx["sha256"],
            "oracle_seconds_one_core": round(dt, 1)}
-    json.dump(out, open(os.path.join(HERE, "sieve_work_config2.json"), "w"), indent=1)
+    json.dump(out, open(os.path.join(HERE, "sieve_work_m12_window_2e24.json"), "w"), indent=1)
     print(json.dumps(out))
 
 

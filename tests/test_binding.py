@@ -6,7 +6,7 @@ its Gerbicz-Li products (cyclo/src/cyclo.h:252-308, tests/host/cyclo_binding_mai
 
 - CPU: the binding compiles against the header, links to the library and
   loads (no device call).
-- GPU: 64 lanes at n = 2^10 (the reference's VSIZE and config 1's size):
+- GPU: 64 lanes at n = 2^10 (the reference's VSIZE, at the size of its full-PRP runs):
   RES64 of 2^(N-1) mod N and of its square on sampled lanes against GMP, the
   verdicts, and every lane against the library's own one-call PRP
   (ct_prp_check), which keeps the exponent bits on the device."""

@@ -1,4 +1,4 @@
-"""Config 4 at its measured shape: 256 candidates b^(2^14) - b^(2^13) + 1,
+"""The batched workload at its measured shape: 256 candidates b^(2^14) - b^(2^13) + 1,
 b uniform in [2^18, 2^19) from the bench's seed, every lane checked against GMP
 (tests/golden/batched_n2e14_x256.json, written by gen_batched_fixture.py):
 RES64 and x mod three 31-bit primes of the canonical residue at

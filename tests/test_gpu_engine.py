@@ -97,7 +97,7 @@ def test_run_matches_oracle_prp_loop(ln):
 
 
 @pytest.mark.parametrize("kind", [1, 2])
-def test_prp_check_config1(kind):
+def test_prp_check_n2e10_batch(kind):
     """Full base-2 PRP with Gerbicz-Li at n = 1024, b even in [2^19, 2^20)."""
     n, ln = 1024, 10
     rng = random.Random(20261203)

@@ -107,7 +107,7 @@ def test_driver_results_and_resume(tmp_path):
     assert (b / f"prp_{ln}.txt").read_text() == (a / f"prp_{ln}.txt").read_text()
 
 
-def test_driver_config1_size(tmp_path):
+def test_driver_n2e10_batch(tmp_path):
     """n = 2^10 digits (the plain PRP configuration), one batch: RES64 lines equal pow."""
     ln = 10
     rng = random.Random(20261203)

@@ -1,4 +1,4 @@
-"""Config 3: one candidate with n = 2^21 digits, b = 65536 + 2u (b = 73764),
+"""One candidate with n = 2^21 digits, b = 65536 + 2u (b = 73764),
 10,000 PRP-loop squarings on the plain engine.  Checked against GMP residues
 (RES64 and x mod three primes, every 1000 squarings, written by
 tests/golden/gen_large_residues.c) and, over all 10,000 squarings, against the

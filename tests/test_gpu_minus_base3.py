@@ -113,8 +113,8 @@ def test_minus_resident_and_multipass_agree():
 
 
 @pytest.mark.parametrize("sign,base,kind", [(-1, 2, 1), (-1, 2, 2), (1, 3, 1), (1, 3, 2), (-1, 3, 1)])
-def test_prp_check_config1_modes(sign, base, kind):
-    """Config-1 shape (n = 1024, b even in [2^19, 2^20)): base^(N-1) mod N with
+def test_prp_check_n2e10_modes(sign, base, kind):
+    """Full-PRP shape at n = 1024 (b even in [2^19, 2^20)): base^(N-1) mod N with
     Gerbicz-Li, for N- and for base 3."""
     n, ln = 1024, 10
     rng = random.Random(20261203)

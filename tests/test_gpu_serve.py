@@ -1,4 +1,4 @@
-"""Serving loop (config 5): continuous batching of mixed sizes.  Every result
+"""Serving loop: continuous batching of mixed sizes.  Every result
 must equal the same request run alone, and Python big-integer pow for the
 sizes where that is quick."""
 import random

@@ -1,4 +1,4 @@
-"""Serving loop at config 5's large classes: n = 2^17 .. 2^20 digits beside
+"""Serving loop at its large classes: n = 2^17 .. 2^20 digits beside
 2^12 .. 2^16, 1000 squarings per request, b of 16 .. 22 bits.  Each result is
 the same in the mix as alone, and the eight fixture requests
 (tests/golden/serve_n2e17_2e20.json, GMP) give GMP's RES64 in the mix; the
@@ -47,7 +47,7 @@ def requests(FIX):
 def test_serve_large_classes_mixed_alone_and_gmp():
     FIX = fixture()
     reqs = requests(FIX)
-    print(f"\n[case] serve config-5 classes: {[(r[1], r[2]) for r in reqs]}", flush=True)
+    print(f"\n[case] serve large classes: {[(r[1], r[2]) for r in reqs]}", flush=True)
     s = ct.Server(hbm_budget=24 << 30, slots=2, chunk=125, max_steps=1000)
     mixed = {}
     pending = list(reqs)

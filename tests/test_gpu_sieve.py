@@ -59,18 +59,18 @@ def test_gpu_sieve_accumulates_into_struck():
     assert np.array_equal(got, ref | pre)
 
 
-def test_gpu_sieve_config2_full_window():
+def test_gpu_sieve_m12_full_window():
     """m = 12, b in [2^20, 2^20 + 2^24), every p < 2^32 on both sides."""
-    path = os.path.join(HERE, "golden", "sieve_config2.json")
+    path = os.path.join(HERE, "golden", "sieve_m12_window_2e24.json")
     if not os.path.exists(path):
-        pytest.skip("config-2 fixture not generated")
+        pytest.skip("m = 12 window fixture not generated")
     c = json.load(open(path))
-    print(f"\n[case] sieve config2", flush=True)
+    print(f"\n[case] sieve m = 12 window", flush=True)
     t0 = time.perf_counter()
     st = ct.sieve_window(c["m"], c["b_min"], c["b_max"], c["p_max_pos"], c["p_max_neg"])
     dt = time.perf_counter() - t0
     got = ct.survivors(st, c["b_min"])
-    print(f"[sieve] config2: {len(got)} survivors in {dt:.3f} s", flush=True)
+    print(f"[sieve] m = 12 window: {len(got)} survivors in {dt:.3f} s", flush=True)
     assert len(got) == c["count"]
     assert [int(x) for x in got[:32]] == c["head"] and [int(x) for x in got[-32:]] == c["tail"]
     text = "".join(f"{int(b)}\n" for b in got)

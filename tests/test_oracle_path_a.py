@@ -218,8 +218,8 @@ def test_prp_matches_pow(n, csize):
         assert prm == (want == 1)
 
 
-def test_prp_c1_size():
-    """One base-2 PRP at the config-1 shape (n = 1024, b even in [2^19, 2^20))."""
+def test_prp_n2e10_size():
+    """One base-2 PRP at n = 1024 digits (b even in [2^19, 2^20))."""
     rng = random.Random(1)
     b = rng.randrange(1 << 18, 1 << 19) * 2
     n = 1024
