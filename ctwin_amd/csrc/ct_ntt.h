@@ -460,6 +460,29 @@ __device__ __forceinline__ void final_inv(uint32_t (&u)[E], uint32_t (&vv)[E], u
 // traffic of one prime's exchange is in flight while the other prime's pass
 // computes.  Every element sees the same butterflies as in the single-prime
 // chains (roots read where they are used), so the residues are identical.
+//
+// A workgroup-wide exchange (the first pass's, when the instance spans more
+// than one wavefront) is skewed between the primes: prime 0's plane is
+// written, prime 1's pass runs, and after the barrier prime 0 is read back
+// and prime 1's plane written while prime 0's next pass runs.  Every barrier
+// then has a pass of work in front of it instead of both primes' LDS traffic
+// (n = 2^14: 30.6 -> 29.5 us per squaring step of the transforms).
+
+template <int LNL, int LAY>
+__device__ __forceinline__ void write_plane(const uint32_t (&u)[E], uint32_t *l, uint32_t tt)
+{
+	uint32_t *w = l + lbase<LNL, LAY>(tt);
+#pragma unroll
+	for (int q = 0; q < E; ++q) w[loff<LNL, LAY>(q)] = u[q];
+}
+
+template <int LNL, int LAY>
+__device__ __forceinline__ void read_plane(uint32_t (&u)[E], const uint32_t *l, uint32_t tt)
+{
+	const uint32_t *r = l + lbase<LNL, LAY>(tt);
+#pragma unroll
+	for (int q = 0; q < E; ++q) u[q] = r[loff<LNL, LAY>(q)];
+}
 
 template <int LNL, int FROM, int TO>
 __device__ __forceinline__ void xchg2(uint32_t (&u0)[E], uint32_t (&u1)[E], uint32_t *l0, uint32_t *l1, uint32_t tt)
@@ -495,18 +518,26 @@ __device__ __forceinline__ void fwd_chain2(uint32_t (&u0)[E], uint32_t (&u1)[E],
 	}
 }
 
-template <int LNL, int L, bool PHI6>
+// inverse passes L down to STOP + 1, each preceded by the exchange from layout L+1
+template <int LNL, int L, bool PHI6, int STOP = -1>
 __device__ __forceinline__ void inv_chain2(uint32_t (&u0)[E], uint32_t (&u1)[E], uint32_t *l0, uint32_t *l1,
                                            const tw_entry *twi0, const tw_entry *twi1, uint32_t base, uint32_t tt,
                                            const mtw &M0, const mtw &M1)
 {
-	if constexpr (L >= 0)
+	if constexpr (L > STOP)
 	{
 		xchg2<LNL, L + 1, L>(u0, u1, l0, l1, tt);
 		inv_pass<0, PHI6 && L == 0>(u0, lazy_ptw<LNL, L>(twi0, base, tt), M0);
 		inv_pass<1, PHI6 && L == 0>(u1, lazy_ptw<LNL, L>(twi1, base, tt), M1);
-		inv_chain2<LNL, L - 1, PHI6>(u0, u1, l0, l1, twi0, twi1, base, tt, M0, M1);
+		inv_chain2<LNL, L - 1, PHI6, STOP>(u0, u1, l0, l1, twi0, twi1, base, tt, M0, M1);
 	}
+}
+
+// The first exchange is workgroup-wide and every later one wavefront-local.
+template <int LNL>
+__device__ __forceinline__ constexpr bool skew_first()
+{
+	return geo<LNL>::a >= 2 && !wave_local<LNL, 0>() && wave_local<LNL, 1>();
 }
 
 // Forward passes, final pass (pointwise square) and inverse passes of primes 0
@@ -517,7 +548,24 @@ __device__ __forceinline__ void square2(uint32_t (&u0)[E], uint32_t (&u1)[E], ui
 {
 	constexpr uint32_t n = geo<LNL>::n;
 	const tw_entry *twf0 = tw, *twi0 = tw + n, *twf1 = tw + 2 * n, *twi1 = tw + 3 * n;
-	fwd_chain2<LNL, 0, PHI6>(u0, u1, l0, l1, twf0, twf1, base, tt);
+	if constexpr (skew_first<LNL>())
+	{   // layout 0 -> 1 skewed; the barrier ahead of prime 0's write orders it
+		// after every earlier access of l0 by the workgroup
+		fwd_pass<0, PHI6>(u0, lazy_ptw<LNL, 0>(twf0, base, tt));
+		__syncthreads();
+		write_plane<LNL, 0>(u0, l0, tt);
+		fwd_pass<1, PHI6>(u1, lazy_ptw<LNL, 0>(twf1, base, tt));
+		__syncthreads();
+		read_plane<LNL, 1>(u0, l0, tt);
+		write_plane<LNL, 0>(u1, l1, tt);
+		fwd_pass<0, false>(u0, lazy_ptw<LNL, 1>(twf0, base, tt));
+		__syncthreads();
+		read_plane<LNL, 1>(u1, l1, tt);
+		fwd_pass<1, false>(u1, lazy_ptw<LNL, 1>(twf1, base, tt));
+		xchg2<LNL, 1, 2>(u0, u1, l0, l1, tt);
+		fwd_chain2<LNL, 2, PHI6>(u0, u1, l0, l1, twf0, twf1, base, tt);
+	}
+	else fwd_chain2<LNL, 0, PHI6>(u0, u1, l0, l1, twf0, twf1, base, tt);
 	mtw M0, M1;
 	if constexpr (PHI6) { M0 = load_mtw(twi0); M1 = load_mtw(twi1); }
 	uint32_t none[E];
@@ -527,7 +575,24 @@ __device__ __forceinline__ void square2(uint32_t (&u0)[E], uint32_t (&u1)[E], ui
 	ftw_lazy<LNL> F1; F1.twf = twf1; F1.twi = twi1; F1.base = base; F1.k0 = E * tt;
 	final_pass<LNL, 0, false>(u0, none, F0);
 	final_pass<LNL, 1, false>(u1, none, F1);
-	inv_chain2<LNL, geo<LNL>::a - 1, PHI6>(u0, u1, l0, l1, twi0, twi1, base, tt, M0, M1);
+	if constexpr (skew_first<LNL>())
+	{   // layout 1 -> 0 skewed: the layout-1 writes land in each wavefront's
+		// own range, which only that wavefront has touched since the last
+		// workgroup barrier
+		inv_chain2<LNL, geo<LNL>::a - 1, PHI6, 1>(u0, u1, l0, l1, twi0, twi1, base, tt, M0, M1);
+		xchg2<LNL, 2, 1>(u0, u1, l0, l1, tt);
+		inv_pass<0, false>(u0, lazy_ptw<LNL, 1>(twi0, base, tt), M0);
+		write_plane<LNL, 1>(u0, l0, tt);
+		inv_pass<1, false>(u1, lazy_ptw<LNL, 1>(twi1, base, tt), M1);
+		__syncthreads();
+		read_plane<LNL, 0>(u0, l0, tt);
+		write_plane<LNL, 1>(u1, l1, tt);
+		inv_pass<0, PHI6>(u0, lazy_ptw<LNL, 0>(twi0, base, tt), M0);
+		__syncthreads();
+		read_plane<LNL, 0>(u1, l1, tt);
+		inv_pass<1, PHI6>(u1, lazy_ptw<LNL, 0>(twi1, base, tt), M1);
+	}
+	else inv_chain2<LNL, geo<LNL>::a - 1, PHI6>(u0, u1, l0, l1, twi0, twi1, base, tt, M0, M1);
 }
 
 // Garner CRT of the three engine residues to a signed 96-bit integer in
