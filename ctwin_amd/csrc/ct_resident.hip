@@ -147,10 +147,12 @@ __device__ __forceinline__ void step(int32_t *D, const int32_t (&yd)[E], bool du
 	// to the carry layout: primes 0 and 1 share one transpose, prime 1 through
 	// D (every read of the digits came before the first barrier of prime 1's
 	// transform; each thread later writes its carried digits to exactly the
-	// words of D it reads here)
+	// words of D it reads here).  No barrier ahead of the writes: every
+	// transform ends by reading the thread's own layout-0 words of lds and D,
+	// the words it writes here, which no other thread has touched since the
+	// transform's last workgroup barrier.
 	{
 		uint32_t *Du = reinterpret_cast<uint32_t *>(D);
-		__syncthreads();
 #pragma unroll
 		for (int q = 0; q < E; ++q)
 		{
