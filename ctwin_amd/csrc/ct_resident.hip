@@ -235,8 +235,9 @@ __device__ __forceinline__ void step_h(int32_t *D, uint32_t P, bool dup, int bas
 	uint32_t r[E];
 	if (P == 0) transform<LN, 0, 2, false, MINUS>(D, none, r, lds[0], tw, t);
 	else transform<LN, 1, 2, false, MINUS>(D, none, r, lds[1], tw, t);
-	// both primes' residues, layout 0, each in its own region
-	__syncthreads();
+	// both primes' residues, layout 0, each in its own region (no barrier
+	// ahead: the transform ended by reading this thread's own layout-0 words
+	// of its region, the words written here)
 	{
 		uint32_t *w = lds[P] + lbase<LN, 0>(t);
 #pragma unroll
