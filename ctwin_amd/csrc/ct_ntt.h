@@ -89,12 +89,18 @@ __device__ __forceinline__ uint32_t elem(uint32_t tt, int q)
 	}
 }
 
-template <int LNL, int FROM, int TO, typename V>
+// Workgroup transpose from layout FROM to layout TO through lds.  In every
+// layout the threads own disjoint sets of words, so the leading barrier (which
+// orders the writes after the workgroup's earlier accesses of lds) can go
+// (LEAD = false) when each thread's last access of lds since the previous
+// workgroup barrier was a read of its own FROM-layout words -- the TO read of
+// the exchange before it in a chain.
+template <int LNL, int FROM, int TO, bool LEAD = true, typename V>
 __device__ __forceinline__ void xchg(V (&u)[E], V *lds, uint32_t tt)
 {
 	V *wr = lds + lbase<LNL, FROM>(tt);
 	V *rd = lds + lbase<LNL, TO>(tt);
-	__syncthreads();
+	if constexpr (LEAD) __syncthreads();
 #pragma unroll
 	for (int q = 0; q < E; ++q) wr[loff<LNL, FROM>(q)] = u[q];
 	__syncthreads();
@@ -119,7 +125,7 @@ __device__ __forceinline__ constexpr bool wave_local()
 // workgroup barriers reduce to ordering within the wavefront (its LDS
 // operations execute in order; wave_barrier keeps the compiler from moving
 // them across).
-template <int LNL, int FROM, int TO, typename V>
+template <int LNL, int FROM, int TO, bool LEAD = true, typename V>
 __device__ __forceinline__ void xchg_flat(V (&u)[E], V *lds, uint32_t tt)
 {
 	if constexpr (wave_local<LNL, FROM>() && wave_local<LNL, TO>())
@@ -134,7 +140,7 @@ __device__ __forceinline__ void xchg_flat(V (&u)[E], V *lds, uint32_t tt)
 		for (int q = 0; q < E; ++q) u[q] = rd[loff<LNL, TO>(q)];
 		__builtin_amdgcn_wave_barrier();
 	}
-	else xchg<LNL, FROM, TO>(u, lds, tt);
+	else xchg<LNL, FROM, TO, LEAD>(u, lds, tt);
 }
 
 // ---- twiddles of one pass, loaded one pass ahead of their use -------------
@@ -381,8 +387,12 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 // (the last one lands in layout a, the final-pass layout).  PF: the roots of
 // pass L+1 (and finally those of the final pass, into F) are loaded into
 // registers before pass L computes, hiding their latency at the price of live
-// VGPRs; otherwise each pass reads its roots where it uses them.
-template <int LNL, int P, int L, bool PHI6, bool PF>
+// VGPRs; otherwise each pass reads its roots where it uses them.  Every
+// exchange after the first follows the previous one's read of its own words
+// and needs no leading barrier; the first has one unless LEAD0 is false (lds
+// untouched by the workgroup since its last barrier, or last read by this
+// thread in layout 0 only).
+template <int LNL, int P, int L, bool PHI6, bool PF, bool LEAD0>
 __device__ __forceinline__ void fwd_chain(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twf, const tw_entry *twi,
                                           uint32_t base, uint32_t tt, const ptw &cur, ftw<LNL> &F)
 {
@@ -397,23 +407,25 @@ __device__ __forceinline__ void fwd_chain(uint32_t (&u)[E], uint32_t *lds, const
 			fwd_pass<P, PHI6 && L == 0>(u, cur);
 		}
 		else fwd_pass<P, PHI6 && L == 0>(u, lazy_ptw<LNL, L>(twf, base, tt));
-		xchg_flat<LNL, L, L + 1>(u, lds, tt);
-		if constexpr (!last) fwd_chain<LNL, P, L + 1, PHI6, PF>(u, lds, twf, twi, base, tt, nxt, F);
+		xchg_flat<LNL, L, L + 1, (L == 0 && LEAD0)>(u, lds, tt);
+		if constexpr (!last) fwd_chain<LNL, P, L + 1, PHI6, PF, LEAD0>(u, lds, twf, twi, base, tt, nxt, F);
 	}
 }
 
 // Forward passes of one instance from layout 0.
-template <int LNL, int P, bool PHI6, bool PF>
+template <int LNL, int P, bool PHI6, bool PF, bool LEAD0 = true>
 __device__ __forceinline__ void fwd_all(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twf, const tw_entry *twi,
                                         uint32_t base, uint32_t tt, ftw<LNL> &F)
 {
 	static_assert(geo<LNL>::a > 0, "instances have at least one radix-16 pass");
 	ptw c0;
 	if constexpr (PF) load_ptw<LNL, 0>(c0, twf, base, tt);
-	fwd_chain<LNL, P, 0, PHI6, PF>(u, lds, twf, twi, base, tt, c0, F);
+	fwd_chain<LNL, P, 0, PHI6, PF, LEAD0>(u, lds, twf, twi, base, tt, c0, F);
 }
 
-// inverse passes L..0, each preceded by the transpose from layout L+1
+// inverse passes L..0, each preceded by the transpose from layout L+1 (the
+// first follows the forward chain's read of layout a, each later one the
+// previous exchange's read: no leading barriers)
 template <int LNL, int P, int L, bool PHI6, bool PF>
 __device__ __forceinline__ void inv_chain(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twi, uint32_t base, uint32_t tt,
                                           const ptw &cur, const mtw &M)
@@ -422,7 +434,7 @@ __device__ __forceinline__ void inv_chain(uint32_t (&u)[E], uint32_t *lds, const
 	{
 		ptw nxt;
 		if constexpr (PF && L > 0) load_ptw<LNL, L - 1>(nxt, twi, base, tt);
-		xchg_flat<LNL, L + 1, L>(u, lds, tt);
+		xchg_flat<LNL, L + 1, L, false>(u, lds, tt);
 		if constexpr (PF) inv_pass<P, PHI6 && L == 0>(u, cur, M);
 		else inv_pass<P, PHI6 && L == 0>(u, lazy_ptw<LNL, L>(twi, base, tt), M);
 		if constexpr (L > 0) inv_chain<LNL, P, L - 1, PHI6, PF>(u, lds, twi, base, tt, nxt, M);

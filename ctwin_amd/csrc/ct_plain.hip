@@ -274,7 +274,7 @@ __device__ __forceinline__ void rowB_prime(plain_args &A, uint32_t *lds, uint32_
 #pragma unroll
 	for (int q = 0; q < E; ++q) u[q] = row[t + q * T];
 	ftw<CL> F;
-	fwd_all<CL, P, false, true>(u, lds, twf, twi, base, t, F);
+	fwd_all<CL, P, false, true, false>(u, lds, twf, twi, base, t, F);     // lds untouched so far
 	if (MUL)
 	{
 		const uint32_t *row2 = A.res2 + P * plane + (size_t)lane * n + (size_t)r * C;

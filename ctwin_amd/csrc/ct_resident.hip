@@ -52,7 +52,10 @@ __device__ __forceinline__ void transform(const int32_t *D, const int32_t (&yd)[
 #pragma unroll
 	for (int k = 0; k < E; ++k) u[k] = from_signed_np<NP>(dr[loff<LN, 0>(k)], p);
 	ftw<LN> F;
-	fwd_all<LN, P, true, false>(u, lds, twf, twi, 1u, t, F);
+	// no leading barrier: the caller's last barrier follows every access of
+	// lds by other threads, and an earlier prime's transform ends by reading
+	// this thread's own layout-0 words
+	fwd_all<LN, P, true, false, false>(u, lds, twf, twi, 1u, t, F);
 	if (MUL)
 	{
 #pragma unroll
