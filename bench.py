@@ -48,16 +48,16 @@ def mul32_alg(n, primes=3):
 # copy kernel of known size: FETCH x 1.994, WRITE x 1.000).  The resident
 # kernel reads its state and tables once per launch whatever the number of
 # squarings; the n = 2^21 figures are per squaring (row pass + fused column pass).
-TRAFFIC_RESIDENT_LAUNCH = (24433 * 1.994 + 45056) * 1024     # k_res_steps<14, 2>, 256 lanes
-TRAFFIC_N2E21_SQUARING = (33170 * 1.994 + 16384 + 8629 * 1.994 + 16904) * 1024   # k_rowB + k_colCA2
+TRAFFIC_RESIDENT_LAUNCH = (24441 * 1.994 + 45056) * 1024     # k_res_steps<14, 2>, 256 lanes
+TRAFFIC_N2E21_SQUARING = (33174 * 1.994 + 16384 + 8621 * 1.994 + 16904) * 1024   # k_rowB + k_colCA2
 # VALU issue slots (rocprofv3 --pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU2 on
 # `bench.py --profile` and `--sieve-only`): instructions minus the quad-cycles
 # in which gfx950 issues two VALU instructions at once (mostly v_mov_b32 beside
 # an arithmetic instruction).  One slot takes a SIMD 4 clocks, so the issue
 # ceiling is 1024 SIMDs x 2.4 GHz / 4 = 614.4 G slots/s.
 VALU_SLOT_PEAK_G = 1024 * 2.4 / 4
-VALU_SLOTS_PER_SQUARING = (899907218 - 18339941) / 49                # k_res_steps<14, 2>, 256 lanes: 50- minus 1-squaring launch
-VALU_SLOTS_N2E21_SQUARING = 6932104 + 6455168                   # k_rowB<21, 8> + k_colCA2<21, 8, 32>
+VALU_SLOTS_PER_SQUARING = (901176042 - 18359072) / 49                # k_res_steps<14, 2>, 256 lanes: 50- minus 1-squaring launch
+VALU_SLOTS_N2E21_SQUARING = 6942052 + 6458985                   # k_rowB<21, 8> + k_colCA2<21, 8, 32>
 SIEVE_ROOTS_NEG_SLOTS = 7505592064 - 1826200320                 # k_roots_neg, m = 12 window of 2^24 b, its 26 launches
 # k_roots_neg per launch on that window: the prime list it reads (FETCH x 1.994)
 # and the write-back of the struck byte map (WRITE), both in HBM bytes
