@@ -52,6 +52,38 @@ def test_square_dup_digit_exact(ln, np_mode):
         assert np.array_equal(got, ref), f"step {it}: first mismatch at {np.argwhere(got != ref)[:4]}"
 
 
+@pytest.mark.parametrize("ln,halves", [(11, False), (12, False), (13, False), (14, False),
+                                       (11, True), (12, True), (13, True)])
+def test_two_prime_resident_forms_digit_exact(ln, halves):
+    """Both two-prime resident forms (primes pass by pass with the skewed
+    workgroup-wide exchange, and primes in workgroup halves) against the
+    oracle's digits, step by step."""
+    n = 1 << ln
+    rng = random.Random(900 + ln)
+    lanes = 3
+    bs = [rng.randrange(1 << 14, 1 << 16) for _ in range(lanes)]
+    print(f"\n[case] two-prime resident ln={ln} halves={halves} bs={bs}", flush=True)
+    t = ct.Transform(ln, lanes, ct.KIND_RESIDENT)
+    t.set_b(bs)
+    assert t.primes == 2
+    t.set_prime_halves(halves)
+    assert t.tuning()[1] == halves
+    x = rand_state(rng, lanes, n, bs)
+    t.write_state(0, x)
+    oes = oracle_engines(n, bs)
+    ref = x.copy()
+    for it in range(4):
+        dup = [rng.randrange(2) for _ in range(lanes)]
+        t.square_dup(dup)
+        for l in range(lanes):
+            row = np.ascontiguousarray(ref[l])
+            oes[l].square_dup(row, dup[l])
+            ref[l] = row
+        got = t.read_state(0)
+        assert np.array_equal(got, ref), f"step {it}: first mismatch at {np.argwhere(got != ref)[:4]}"
+    t.close()
+
+
 @pytest.mark.parametrize("ln", [5, 10, 13, 14])
 def test_mul_digit_exact(ln):
     n = 1 << ln
