@@ -294,29 +294,34 @@ def sieve_pool(ct, m, bits, cache):
     return cache[key]
 
 
-def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30, graphs=True):
+def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30, graphs=True, twin=False):
     """Served stream: Poisson arrivals, n = 2^k with k uniform in 12..20, b of 16..22
     bits taken from sieve survivors, 1000 squarings per request, admission by an
-    HBM budget."""
+    HBM budget.  twin: each arrival is a candidate pair, its N+ and N- tests
+    submitted together (requests / 2 pairs)."""
     import numpy as np
     rng = random.Random(SEED + 5)
     pools = {}
     t_arr, reqs, t = [], [], 0.0
-    for i in range(requests):
-        t += rng.expovariate(rate)
+    signs = (1, -1) if twin else (1,)
+    for i in range(requests // len(signs)):
+        t += rng.expovariate(rate / len(signs))
         bits = rng.randrange(16, 23)
         k = rng.randrange(12, 21)
-        reqs.append((i, k, rng.choice(sieve_pool(ct, k, bits, pools)), steps))
-        t_arr.append(t)
+        b = rng.choice(sieve_pool(ct, k, bits, pools))
+        for sg in signs:
+            reqs.append((len(reqs), k, b, steps, sg))
+            t_arr.append(t)
     s = ct.Server(hbm_budget=budget, slots=16, chunk=125, max_steps=steps)
     s.set_graphs(graphs)
     # warm-up: one short request per size class creates the class engines
     # (autotuned kind and tile, tables, buffers) as a long-running server would
     # already hold them
     for k in range(12, 21):
-        s.submit(10 ** 6 + k, k, 65537, 8)
+        for sg in signs:
+            s.submit(10 ** 6 + 2 * k + (sg < 0), k, 65537, 8, sg)
     warm = 0
-    while warm < 9:
+    while warm < 9 * len(signs):
         s.step()
         warm += len(s.poll())
     sq0 = s.stats()["squarings"]
@@ -338,11 +343,13 @@ def run_served(ct, requests=72, steps=1000, rate=400.0, budget=16 << 30, graphs=
     L = np.array(sorted(lat.values()))
     per_class = {}
     for k in range(12, 21):
-        v = np.array([lat[r[0]] for r in reqs if r[1] == k])
-        if len(v):
-            per_class[f"2^{k}"] = {"requests": int(len(v)), "p50_ms": 1e3 * float(np.percentile(v, 50)),
+        for sg in signs:
+            v = np.array([lat[r[0]] for r in reqs if r[1] == k and r[4] == sg])
+            if len(v):
+                name = f"2^{k}" + (("+" if sg > 0 else "-") if twin else "")
+                per_class[name] = {"requests": int(len(v)), "p50_ms": 1e3 * float(np.percentile(v, 50)),
                                    "p99_ms": 1e3 * float(np.percentile(v, 99))}
-    return {"requests": requests, "squarings_per_request": steps, "arrival_rate_per_s": rate,
+    return {"requests": len(reqs), "twin_pairs": twin, "squarings_per_request": steps, "arrival_rate_per_s": rate,
             "wall_s": wall, "squarings_per_s": (st["squarings"] - sq0) / wall,
             "requests_per_s": requests / wall,
             "latency_p50_ms": 1e3 * float(np.percentile(L, 50)), "latency_p99_ms": 1e3 * float(np.percentile(L, 99)),
@@ -519,6 +526,9 @@ def main():
     served_light = run_served(ct, requests=48, rate=40.0, graphs=not args.no_graphs)
     print(f"[bench] served below saturation: {served_light['squarings_per_s']:.4g} squarings/s, "
           f"p50 {served_light['latency_p50_ms']:.1f} ms, p99 {served_light['latency_p99_ms']:.1f} ms", flush=True)
+    served_twin = run_served(ct, twin=True, graphs=not args.no_graphs)
+    print(f"[bench] served twin pairs (N+ and N-): {served_twin['squarings_per_s']:.4g} squarings/s, "
+          f"p50 {served_twin['latency_p50_ms']:.1f} ms, p99 {served_twin['latency_p99_ms']:.1f} ms", flush=True)
 
     prp10 = run_prp_n2e10(ct)
     print(f"[bench] full PRP n = 2^10: {prp10['plus_base3_batched1024']['candidates_per_hour']:.4g} N+ tests/hour (1024 lanes), "
@@ -567,6 +577,7 @@ def main():
         "sieve_m12_window": sieve,
         "served": served,
         "served_below_saturation": served_light,
+        "served_twin_pairs": served_twin,
         "prp_n2e10": prp10,
         "roofline": {"bound": "valu", "achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "Tmul32/s",
                      "frac": valu_achieved / VALU_PEAK_T, "traffic": TRAFFIC_RESIDENT_LAUNCH,
