@@ -389,9 +389,9 @@ __device__ __forceinline__ void final_pass(uint32_t (&u)[E], uint32_t (&vv)[E], 
 // registers before pass L computes, hiding their latency at the price of live
 // VGPRs; otherwise each pass reads its roots where it uses them.  Every
 // exchange after the first follows the previous one's read of its own words
-// and needs no leading barrier; the first has one unless LEAD0 is false (lds
-// untouched by the workgroup since its last barrier, or last read by this
-// thread in layout 0 only).
+// and needs no leading barrier; the first has one unless LEAD0 is false (since
+// the workgroup's last barrier no thread has touched lds, or each has only
+// read its own layout-0 words).
 template <int LNL, int P, int L, bool PHI6, bool PF, bool LEAD0>
 __device__ __forceinline__ void fwd_chain(uint32_t (&u)[E], uint32_t *lds, const tw_entry *twf, const tw_entry *twi,
                                           uint32_t base, uint32_t tt, const ptw &cur, ftw<LNL> &F)
