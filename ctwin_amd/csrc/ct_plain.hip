@@ -351,7 +351,9 @@ __device__ __forceinline__ void rowS_prime(plain_args &A, uint32_t *lds, uint32_
 	const tw_entry *twf = A.tw + (size_t)(P * 2) * n, *twi = A.tw + (size_t)(P * 2 + 1) * n;
 	uint32_t dummy[E];
 	ftw<CL> F;
-	fwd_all<CL, P, false, true>(u, lds, twf, twi, R + r, t, F);
+	// no leading barrier: lds is untouched so far (P = 0), or the barrier
+	// after the previous prime's transposes came last (P > 0)
+	fwd_all<CL, P, false, true, false>(u, lds, twf, twi, R + r, t, F);
 	final_inv<CL, P, false, false, true>(u, dummy, lds, twf, twi, R + r, t, F);
 	uint32_t *row = A.res + P * plane + (size_t)lane * n + (size_t)r * C;
 #pragma unroll
