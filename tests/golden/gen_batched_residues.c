@@ -10,7 +10,8 @@
  * even (b^j = 0 mod 2^64 once j v2(b) >= 64).
  *
  * build: gcc -O2 -I/opt/conda/include gen_batched_residues.c -L/opt/conda/lib -lgmp -o gen_batched
- * run:   ./gen_batched <log2_n> <i0> <t1,t2,...> <b> [<b> ...]
+ * run:   ./gen_batched [-m] <log2_n> <i0> <t1,t2,...> <b> [<b> ...]
+ * -m: the other twin member, N = b^n - b^(n/2) - 1 (E = N - 1 likewise).
  */
 #include <gmp.h>
 #include <stdio.h>
@@ -19,7 +20,9 @@
 
 int main(int argc, char **argv)
 {
-	if (argc < 5) { fprintf(stderr, "usage: gen_batched log2_n i0 t1,t2,... b [b ...]\n"); return 1; }
+	int minus = 0;
+	if (argc > 1 && strcmp(argv[1], "-m") == 0) { minus = 1; ++argv; --argc; }
+	if (argc < 5) { fprintf(stderr, "usage: gen_batched [-m] log2_n i0 t1,t2,... b [b ...]\n"); return 1; }
 	const unsigned ln = (unsigned)atoi(argv[1]);
 	const long i0 = atol(argv[2]);
 	long ts[256];
@@ -35,6 +38,7 @@ int main(int argc, char **argv)
 		mpz_ui_pow_ui(h, b, n / 2);
 		mpz_mul(E, h, h);
 		mpz_sub(E, E, h);
+		if (minus) mpz_sub_ui(E, E, 2);
 		mpz_add_ui(N, E, 1);
 		if ((long)mpz_sizeinbase(E, 2) - 1 > i0) { fprintf(stderr, "b=%lu: E longer than i0\n", b); return 2; }
 		mpz_set_ui(x, 1);
